@@ -1,0 +1,245 @@
+"""ctypes view of the C oracle (oracle/afid_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the afid_amd product package.
+Arrays are returned as numpy views in the reference's Fortran layout
+(x fastest) with the reference's index bases folded to 0:
+  halo fields  vx,vy,vz,pr,temp -> shape (nzm+2, nym+2, nx)   [i, j, k]
+  plain fields rhs,ru*,qcap,dq,hro -> (nzm, nym, nx); dph -> (nzm, nym, nxm)
+  1-D arrays -> length n+2, element 0 unused (Fortran 1-based).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "liboracle.so")
+REF_HARNESS = os.path.join(HERE, "_ref", "ref_harness")
+
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-s", "-C", HERE, "lib"], check=True)
+    return LIB_PATH
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        build()
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+        L.afo_create.restype = vp
+        L.afo_create.argtypes = [i, i, i, i, d, i, d, d, d, d, d, i, i]
+        L.afo_destroy.argtypes = [vp]
+        L.afo_array.restype = vp
+        L.afo_array.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(i)]
+        L.afo_get_scalar.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(d)]
+        L.afo_set_scalar.argtypes = [vp, ctypes.c_char_p, d]
+        L.afo_init_linear_noise.argtypes = [vp, ctypes.c_ulonglong, d]
+        L.afo_init_reference_ic.argtypes = [vp]
+        for name in ("explicit_terms_vx", "explicit_terms_vy", "explicit_terms_vz", "explicit_terms_temp",
+                     "implicit_and_update_vx", "implicit_and_update_vy", "implicit_and_update_vz",
+                     "implicit_and_update_temp", "calc_local_divergence", "copy_dphhalo",
+                     "correct_velocity", "correct_pressure", "update_all_halos"):
+            getattr(L, "afo_" + name).argtypes = [vp]
+        L.afo_solve_pressure_correction.argtypes = [vp]
+        L.afo_solve_pressure_correction.restype = i
+        L.afo_update_halo.argtypes = [vp, ctypes.c_char_p]
+        L.afo_time_marcher.argtypes = [vp, d]
+        L.afo_time_marcher.restype = i
+        L.afo_substep.argtypes = [vp, i, d]
+        L.afo_substep.restype = i
+        L.afo_step.argtypes = [vp, i, i, d, d, d, d, ctypes.POINTER(d)]
+        L.afo_step.restype = i
+        L.afo_calc_max_cfl.argtypes = [vp]
+        L.afo_calc_max_cfl.restype = d
+        L.afo_check_divergence.argtypes = [vp]
+        L.afo_check_divergence.restype = d
+        L.afo_calc_plate_nu.argtypes = [vp, ctypes.POINTER(d), ctypes.POINTER(d)]
+        L.afo_global_quantities.argtypes = [vp, ctypes.POINTER(d)]
+        L.afo_calc_dissipation_nu.argtypes = [vp, ctypes.POINTER(d), ctypes.POINTER(d),
+                                              ctypes.POINTER(d), ctypes.POINTER(d)]
+        L.afo_fft_c2c.argtypes = [i, ctypes.POINTER(d), ctypes.POINTER(d), i]
+        L.afo_fft_r2c.argtypes = [i, ctypes.POINTER(d), ctypes.POINTER(d)]
+        L.afo_fft_c2r.argtypes = [i, ctypes.POINTER(d), ctypes.POINTER(d)]
+        L.afo_dgttrf.argtypes = [i] + [ctypes.POINTER(d)] * 4 + [ctypes.POINTER(i)]
+        L.afo_dgttrf.restype = i
+        L.afo_dgttrs.argtypes = [i, i] + [ctypes.POINTER(d)] * 4 + [ctypes.POINTER(i), ctypes.POINTER(d), i]
+        L.afo_set_threads.argtypes = [i]
+        _lib = L
+    return _lib
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Oracle:
+    """One reference-state instance (module globals of param/local_arrays)."""
+
+    def __init__(self, nxm, nym, nzm, nsst=3, alx3=1.0, istr3=0, str3=0.0, ylen=1.0, zlen=1.0,
+                 ray=1e6, pra=1.0, inslws=1, inslwn=1):
+        self.L = lib()
+        self.nxm, self.nym, self.nzm = nxm, nym, nzm
+        self.nx = nxm + 1
+        self.params = dict(nxm=nxm, nym=nym, nzm=nzm, nsst=nsst, alx3=alx3, istr3=istr3, str3=str3,
+                           ylen=ylen, zlen=zlen, ray=ray, pra=pra, inslws=inslws, inslwn=inslwn)
+        self.h = self.L.afo_create(nxm, nym, nzm, nsst, alx3, istr3, str3, ylen, zlen, ray, pra,
+                                   inslws, inslwn)
+
+    def close(self):
+        if self.h:
+            self.L.afo_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- array access -----------------------------------------------------
+    _halo = {"vx", "vy", "vz", "pr", "temp"}
+    _plain = {"rhs", "rux", "ruy", "ruz", "rutemp", "qcap", "dq", "hro"}
+
+    def arr(self, name: str) -> np.ndarray:
+        cnt = ctypes.c_long()
+        isint = ctypes.c_int()
+        p = self.L.afo_array(self.h, name.encode(), ctypes.byref(cnt), ctypes.byref(isint))
+        if not p:
+            raise KeyError(name)
+        ct = ctypes.c_int if isint.value else ctypes.c_double
+        a = np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ct)), shape=(cnt.value,))
+        nx, nxm, nym, nzm = self.nx, self.nxm, self.nym, self.nzm
+        if name in self._halo:
+            return a.reshape(nzm + 2, nym + 2, nx)
+        if name == "dphhalo":
+            return a.reshape(nzm + 2, nym + 2, nxm)
+        if name in self._plain:
+            return a.reshape(nzm, nym, nx)
+        if name == "dph":
+            return a.reshape(nzm, nym, nxm)
+        if name in ("tempbp", "temptp"):
+            return a.reshape(nzm, nym)
+        return a
+
+    def interior(self, name: str) -> np.ndarray:
+        """Copy of a halo field without halos: shape (nzm, nym, nx)."""
+        return self.arr(name)[1:-1, 1:-1, :].copy()
+
+    def set_interior(self, name: str, values: np.ndarray):
+        self.arr(name)[1:-1, 1:-1, :] = values
+        self.L.afo_update_halo(self.h, name.encode())
+
+    def scalar(self, name: str) -> float:
+        v = ctypes.c_double()
+        if self.L.afo_get_scalar(self.h, name.encode(), ctypes.byref(v)):
+            raise KeyError(name)
+        return v.value
+
+    def set_scalar(self, name: str, value: float):
+        if self.L.afo_set_scalar(self.h, name.encode(), float(value)):
+            raise KeyError(name)
+
+    def set_substep(self, ns: int, dt: float):
+        """Coefficients as TimeMarcher.F90:22-30 sets them for substep ns."""
+        self.set_scalar("dt", dt)
+        self.set_scalar("beta", dt / self.scalar("ren") * 0.5)
+        self.set_scalar("al", self.scalar(f"alm{ns}"))
+        self.set_scalar("ga", self.scalar(f"gam{ns}"))
+        self.set_scalar("ro", self.scalar(f"rom{ns}"))
+
+    # ---- routines ----------------------------------------------------------
+    def init_linear_noise(self, seed: int, amp: float = 1e-3):
+        self.L.afo_init_linear_noise(self.h, seed, amp)
+
+    def init_reference_ic(self):
+        self.L.afo_init_reference_ic(self.h)
+
+    def call(self, routine: str):
+        fn = getattr(self.L, "afo_" + routine)
+        return fn(self.h)
+
+    def substep(self, ns: int, dt: float) -> int:
+        return self.L.afo_substep(self.h, ns, dt)
+
+    def time_marcher(self, dt: float) -> int:
+        return self.L.afo_time_marcher(self.h, dt)
+
+    def step(self, ntime, idtv, dt, dtmin, dtmax, limitcfl):
+        used = ctypes.c_double()
+        err = self.L.afo_step(self.h, ntime, idtv, dt, dtmin, dtmax, limitcfl, ctypes.byref(used))
+        return err, used.value
+
+    def cfl(self) -> float:
+        return self.L.afo_calc_max_cfl(self.h)
+
+    def divmax(self) -> float:
+        return self.L.afo_check_divergence(self.h)
+
+    def plate_nu(self):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        self.L.afo_calc_plate_nu(self.h, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+    def global_quantities(self) -> np.ndarray:
+        out = np.zeros(11)
+        self.L.afo_global_quantities(self.h, _dp(out))
+        return out
+
+    def dissipation(self):
+        mu, th = ctypes.c_double(), ctypes.c_double()
+        de = np.zeros(self.nxm)
+        dt_ = np.zeros(self.nxm)
+        self.L.afo_calc_dissipation_nu(self.h, ctypes.byref(mu), ctypes.byref(th), _dp(de), _dp(dt_))
+        return mu.value, th.value, de, dt_
+
+
+def fft_c2c(x: np.ndarray, sign: int = -1) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.complex128)
+    out = np.zeros_like(x)
+    lib().afo_fft_c2c(x.size, _dp(x.view(np.float64)), _dp(out.view(np.float64)), sign)
+    return out
+
+
+def fft_r2c(x: np.ndarray) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros(x.size // 2 + 1, dtype=np.complex128)
+    lib().afo_fft_r2c(x.size, _dp(x), _dp(out.view(np.float64)))
+    return out
+
+
+def fft_c2r(c: np.ndarray, n: int) -> np.ndarray:
+    c = np.ascontiguousarray(c, dtype=np.complex128)
+    out = np.zeros(n)
+    lib().afo_fft_c2r(n, _dp(c.view(np.float64)), _dp(out))
+    return out
+
+
+# ---- reference harness (oracle/_ref/ref_harness, built only where the
+# ---- reference source tree is present) ------------------------------------
+def ref_available() -> bool:
+    return os.path.exists(REF_HARNESS)
+
+
+def run_ref(workdir: str, params: dict, mode: int, nsteps: int = 1, dt: float = 1e-3, idtv: int = 0,
+            dtmin: float = 1e-8, dtmax: float = 1e-2, limitcfl: float = 1.2, timeout: float = 600):
+    p = params
+    with open(os.path.join(workdir, "harness.in"), "w") as f:
+        f.write(f"{p['nxm']} {p['nym']} {p['nzm']} {p['nsst']}\n")
+        f.write(f"{p['alx3']!r} {p['istr3']} {p['str3']!r}\n")
+        f.write(f"{p['ylen']!r} {p['zlen']!r}\n")
+        f.write(f"{p['ray']!r} {p['pra']!r}\n")
+        f.write(f"{p['inslws']} {p['inslwn']}\n")
+        f.write(f"{mode} {nsteps} {dt!r} {idtv} {dtmin!r} {dtmax!r} {limitcfl!r}\n")
+    env = dict(os.environ)
+    env.setdefault("OMP_NUM_THREADS", "1")
+    subprocess.run([REF_HARNESS], cwd=workdir, check=True, env=env, timeout=timeout,
+                   stdout=subprocess.DEVNULL, stderr=subprocess.PIPE)
