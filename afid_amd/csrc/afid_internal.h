@@ -1,0 +1,70 @@
+// afid_internal.h -- device data layout and kernel argument blocks shared by
+// the HIP translation units of libafid_hip.
+//
+// Field layout in HBM (all FP64, one contiguous block per field):
+//   column (j,i), j in [0,nym), i in [0,nzm), starts at ((i*nym)+j)*ldx.
+//   Inside a column the nx wall-normal points are chunk-interleaved: the
+//   column is cut into P=16 chunks of L=ceil(nx/16) points, and point k is
+//   stored at pos(k) = (k mod L)*16 + k div L, ldx = 16*L.  A wave owns four
+//   columns with one lane per chunk, so lane p's t-th point of all four
+//   columns sits in one 128-byte run: every load of the column kernels is
+//   coalesced, and the wall-normal tridiagonal solves stay in registers.
+//   Periodic y/z neighbours are reached by index wrap; there are no halos.
+//
+// Spectral array (pressure solve): S[k][n][m] complex128, k in [0,nxm),
+// n in [0,nzm) (z wavenumber), m in [0,ldm) (y wavenumber, m<nym/2+1 used),
+// m fastest.  x-solves walk k with one thread per (m,n) mode.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace afid {
+
+constexpr int P = 16;        // chunk lanes per column
+constexpr int COLS_PER_WG = 16;  // 256 threads = 4 waves x 4 columns
+
+struct Geom {
+  int nx, nxm, nym, nzm;
+  int L, ldx;       // chunk length, column pitch (16*L)
+  int64_t ncol;     // nym*nzm
+};
+
+__host__ __device__ inline int pos_of_k(int k, int L) { return (k % L) * P + k / L; }
+__host__ __device__ inline int k_of_pos(int s, int L) { return (s % P) * L + s / P; }
+__host__ __device__ inline int64_t col_off(int j, int i, const Geom &g) {
+  return ((int64_t)i * g.nym + j) * g.ldx;
+}
+__host__ __device__ inline int wrapm(int a, int n) { return a < 0 ? a + n : a; }
+__host__ __device__ inline int wrapp(int a, int n) { return a >= n ? a - n : a; }
+
+// Per-k coefficient rows held on device in the interleaved layout: row q of
+// the block kc is kc[q*ldx + pos(k)].
+enum KCoef {
+  KC_UDX3C = 0, KC_UDX3M, KC_AM3CK, KC_AC3CK, KC_AP3CK, KC_AM3SK, KC_AC3SK, KC_AP3SK,
+  KC_AMPHK, KC_ACPHK, KC_APPHK, KC_KMVPOS, KC_KPVPOS, KC_N
+};
+
+// Factor rows of one wall-normal system (interleaved layout, length ldx):
+// scale = row normalisation 1/(1 - ac*betadx) (SolveImpEqnUpdate_*),
+// l = LU multipliers, rinv = 1/u, e = du/u, G/H = chunk prefix/suffix
+// products of -l / -e used by the lane scan.
+enum FacRow { FR_SCALE = 0, FR_L, FR_RINV, FR_E, FR_G, FR_H, FR_N };
+enum System { SYS_X = 0, SYS_YZ, SYS_T, SYS_N };
+
+struct StepScalars {
+  double al, ga, ro, dt, beta;
+  double alre, alpec;            // al/ren, al/pec
+  double udy4, udz4;             // dy/4, dz/4
+  double udyq_re, udzq_re;       // dyq/ren, dzq/ren
+  double udyq_pe, udzq_pe;       // dyq/pec, dzq/pec
+  double dy, dz, dyq, dzq;
+  double udy_al;                 // dy*al (ImplicitAndUpdateVY.F90:33)
+  double tempbp, temptp;
+  int use_ru;                    // ro != 0
+};
+
+struct FieldPtrs {
+  const double *vx, *vy, *vz, *temp, *pr;
+};
+
+}  // namespace afid

@@ -1,0 +1,123 @@
+// kernels.h -- argument blocks and launchers of the libafid_hip kernels.
+#pragma once
+#include "afid_internal.h"
+
+namespace afid {
+
+struct FusedArgs {
+  Geom g;
+  FieldPtrs f;                       // fields at the start of the substep
+  double *vxn, *vyn, *vzn, *tn;      // updated fields (separate buffers)
+  double *rux, *ruy, *ruz, *rut;     // explicit-term history, updated in place
+  double *stale;                     // rhs(1,j,i) of the reference, nym*nzm
+  const double *kc;                  // KC_N interleaved coefficient rows
+  const double *fac;                 // SYS_N*FR_N interleaved factor rows
+  StepScalars sc;
+};
+
+struct RoutineArgs {
+  Geom g;
+  FieldPtrs f;
+  const double *kc;
+  double *out;          // explicit term / divergence output
+  const double *expl;   // explicit term input of the RHS loop
+  double *ru;           // history array of the RHS loop
+  double *rhs;          // rhs scratch (reference local_arrays rhs)
+  StepScalars sc;
+};
+
+struct SolveArgs {
+  Geom g;
+  const double *fac;    // FR_N rows of this system
+  double *rhs;
+  double *q;            // field updated in place
+  double *stale;
+  double tempbp, temptp;
+};
+
+struct CorrectArgs {
+  Geom g;
+  const double *dph;
+  double *vx, *vy, *vz, *pr;
+  const double *kc;
+  StepScalars sc;
+};
+
+// One complex FFT length with its radix stages, twiddles exp(-2 pi i t/n)
+// and the mixed-radix digit reversal of the in-place decimation-in-time order.
+struct FFTPlan {
+  int n, nf;
+  int f[16];
+  const double2 *tw;
+  const int *rev;
+};
+
+// Real transform of length N along a lateral direction, computed as a complex
+// transform of H = N/2 points (N even) or N points (N odd) in LDS.
+struct RealPlan {
+  int N, H, half;
+  FFTPlan c;
+  const double2 *post;   // exp(-2 pi i m/N), m = 0..N/2
+};
+
+struct PoissonArgs {
+  Geom g;
+  int nymh, ldm;         // y wavenumbers kept (nym/2+1), padded row length
+  double2 *S;            // spectral array S[k][n][m]
+  double *cp;            // x-solve scratch, same indexing as S
+  const double *kc;      // for KC_ACPHK/AMPHK/APPHK (read through kx arrays below)
+  const double *amphk, *acphk, *apphk;  // natural k order, length nxm
+  const double *ak1, *ak2;              // z (nzm) and y (nym) modified wavenumbers
+  FieldPtrs f;           // velocities for the fused divergence
+  double *dph;           // real field (interleaved layout)
+  const double *udx3m;   // interleaved row KC_UDX3M
+  StepScalars sc;
+  RealPlan py;
+  FFTPlan pz;
+  int from_velocity;     // 1: y-FFT input is div(u)/(dt al) computed on the fly; 0: dph
+};
+
+hipError_t launch_fused_update(const FusedArgs &a, hipStream_t st);
+hipError_t launch_solve_update(const SolveArgs &a, int which, hipStream_t st);
+hipError_t launch_explicit(const RoutineArgs &a, int which, hipStream_t st);
+hipError_t launch_rhs(const RoutineArgs &a, int which, hipStream_t st);
+hipError_t launch_divergence(const RoutineArgs &a, hipStream_t st);
+hipError_t launch_correct(const CorrectArgs &a, int which, hipStream_t st);
+int max_supported_L();
+hipError_t column_configure_lds(size_t bytes);
+
+hipError_t launch_yfft_forward(const PoissonArgs &a, hipStream_t st);
+hipError_t launch_zfft(const PoissonArgs &a, int sign, hipStream_t st);
+hipError_t launch_xsolve_pressure(const PoissonArgs &a, hipStream_t st);
+hipError_t launch_yfft_inverse(const PoissonArgs &a, hipStream_t st);
+int fft_max_len();
+hipError_t fft_configure_lds(size_t bytes);
+
+// Reductions: results land in dev_out (double); the caller copies them.
+struct ReduceArgs {
+  Geom g;
+  FieldPtrs f;
+  const double *kc;
+  const double *g3rc;      // natural order, length nx
+  const double *dxc, *dxm, *lem, *lec;  // natural order, length nxm (dissipation)
+  double dy, dz, ren, pec, pra;
+  double *partial;         // scratch, nblk * NQ
+  double *out;
+};
+hipError_t launch_reduce_cfl(const ReduceArgs &a, hipStream_t st);        // out[0]
+hipError_t launch_reduce_divmax(const ReduceArgs &a, hipStream_t st);     // out[0]
+hipError_t launch_reduce_global(const ReduceArgs &a, hipStream_t st);     // out[0..10] raw sums/max
+hipError_t launch_reduce_platenu(const ReduceArgs &a, double del, double deln, hipStream_t st);  // out[0..1]
+hipError_t launch_reduce_dissipation(const ReduceArgs &a, hipStream_t st);  // out[0..1], out[2..] profiles
+int reduce_partial_blocks();
+
+// Layout conversion host-array staging <-> interleaved device field.
+hipError_t launch_gather_in(const double *stage, double *dst, const Geom &g, int n1, int ld1, int ld2,
+                            int j0, int i0, hipStream_t st);
+hipError_t launch_scatter_out(const double *src, double *stage, const Geom &g, int n1, int ld1, int ld2,
+                              int j0, int i0, int halo, hipStream_t st);
+hipError_t launch_init_noise(double *temp, double *zero_fields[4], const double *xc_nat, const Geom &g,
+                             uint64_t seed, double amp, double tbp, double ttp, hipStream_t st);
+hipError_t launch_fill(double *p, double v, int64_t n, hipStream_t st);
+
+}  // namespace afid

@@ -1,0 +1,324 @@
+// misc_kernels.hip -- diagnostics reductions (CalcMaxCFL.F90, CheckDivergence.F90,
+// CalcPlateNu.F90, GlobalQuantities.F90, CalcDissipationNu.F90), layout
+// conversion between the reference's Fortran arrays and the interleaved
+// device layout, and initial conditions.  All reductions are FP64: per-block
+// partials in a fixed order, then one block combines them (deterministic).
+#include "afid_internal.h"
+#include "kernels.h"
+
+namespace afid {
+
+constexpr int RBLK = 1024;  // partial blocks
+int reduce_partial_blocks() { return RBLK; }
+
+template <typename Op>
+__device__ double block_reduce(double v, Op op, double init) {
+  __shared__ double sh[256];
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] = op(sh[threadIdx.x], sh[threadIdx.x + w]);
+    __syncthreads();
+  }
+  double r = sh[0];
+  __syncthreads();
+  (void)init;
+  return r;
+}
+struct OpMax { __device__ double operator()(double a, double b) const { return fmax(a, b); } };
+struct OpMin { __device__ double operator()(double a, double b) const { return fmin(a, b); } };
+struct OpSum { __device__ double operator()(double a, double b) const { return a + b; } };
+
+// ops: 0 sum, 1 max, 2 min for each of nq quantities
+__global__ void k_finalize(const double *partial, double *out, int nq, const int *ops_dummy, int op_mask_max,
+                           int op_mask_min) {
+  const int q = blockIdx.x;
+  const bool ismax = (op_mask_max >> q) & 1, ismin = (op_mask_min >> q) & 1;
+  double v = ismax ? -HUGE_VAL : (ismin ? HUGE_VAL : 0.0);
+  for (int b = threadIdx.x; b < RBLK; b += blockDim.x) {
+    double x = partial[(int64_t)b * nq + q];
+    v = ismax ? fmax(v, x) : (ismin ? fmin(v, x) : v + x);
+  }
+  if (ismax) v = block_reduce(v, OpMax(), 0);
+  else if (ismin) v = block_reduce(v, OpMin(), 0);
+  else v = block_reduce(v, OpSum(), 0);
+  if (threadIdx.x == 0) out[q] = v;
+  (void)ops_dummy;
+}
+
+// iterate over interior points (k < kmax) of all columns, grid-stride
+#define FOR_POINTS(g, kmax)                                                            \
+  for (int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; gid < (g).ncol * (g).ldx; \
+       gid += (int64_t)gridDim.x * blockDim.x)                                          \
+    if (k_of_pos((int)(gid % (g).ldx), (g).L) < (kmax))
+
+__global__ void k_red_cfl(ReduceArgs a) {
+  const Geom g = a.g;
+  double m = 0.0;
+  FOR_POINTS(g, g.nxm) {
+    const int64_t col = gid / g.ldx;
+    const int s = (int)(gid % g.ldx), k = k_of_pos(s, g.L);
+    const int j = (int)(col % g.nym), i = (int)(col / g.nym);
+    const int64_t C = col * g.ldx;
+    const int64_t CJP = col_off(wrapp(j + 1, g.nym), i, g), CIP = col_off(j, wrapp(i + 1, g.nzm), g);
+    const int skp = pos_of_k(k + 1, g.L);
+    double qcf = (fabs((a.f.vz[C + s] + a.f.vz[CIP + s]) * 0.5 * a.dz) +
+                  fabs((a.f.vy[C + s] + a.f.vy[CJP + s]) * 0.5 * a.dy) +
+                  fabs((a.f.vx[C + s] + a.f.vx[C + skp]) * 0.5 * a.kc[KC_UDX3M * g.ldx + s]));
+    m = fmax(m, qcf);
+  }
+  m = block_reduce(m, OpMax(), 0);
+  if (threadIdx.x == 0) a.partial[blockIdx.x] = m;
+}
+
+__global__ void k_red_div(ReduceArgs a) {
+  const Geom g = a.g;
+  double m = 0.0;
+  FOR_POINTS(g, g.nxm) {
+    const int64_t col = gid / g.ldx;
+    const int s = (int)(gid % g.ldx), k = k_of_pos(s, g.L);
+    const int j = (int)(col % g.nym), i = (int)(col / g.nym);
+    const int64_t C = col * g.ldx;
+    const int64_t CJP = col_off(wrapp(j + 1, g.nym), i, g), CIP = col_off(j, wrapp(i + 1, g.nzm), g);
+    const int skp = pos_of_k(k + 1, g.L);
+    double dq = (a.f.vz[CIP + s] - a.f.vz[C + s]) * a.dz + (a.f.vy[CJP + s] - a.f.vy[C + s]) * a.dy +
+                (a.f.vx[C + skp] - a.f.vx[C + s]) * a.kc[KC_UDX3M * g.ldx + s];
+    m = fmax(m, fabs(dq));
+  }
+  m = block_reduce(m, OpMax(), 0);
+  if (threadIdx.x == 0) a.partial[blockIdx.x] = m;
+}
+
+// GlobalQuantities.F90:38-56 sums/extrema: q0 anusin, q1 tempm, q2 vx2, q3 vy2,
+// q4 vz2, q5 all2 (sums); q6..q8 vmax(vz,vy,vx), q9 tempmax (max); q10 tempmin (min)
+__global__ void k_red_global(ReduceArgs a) {
+  const Geom g = a.g;
+  double an = 0, tm = 0, x2 = 0, y2 = 0, z2 = 0, a2 = 0, mz = 0, my = 0, mx = 0, tmax = -HUGE_VAL,
+         tmin = HUGE_VAL;
+  FOR_POINTS(g, g.nxm) {
+    const int64_t col = gid / g.ldx;
+    const int s = (int)(gid % g.ldx), k = k_of_pos(s, g.L);
+    const int64_t C = col * g.ldx;
+    const int skp = pos_of_k(k + 1, g.L);
+    const double fac2 = a.g3rc[k];
+    const double vx = a.f.vx[C + s], vy = a.f.vy[C + s], vz = a.f.vz[C + s], T = a.f.temp[C + s];
+    mz = fmax(mz, fabs(vz));
+    my = fmax(my, fabs(vy));
+    mx = fmax(mx, fabs(vx));
+    tmax = fmax(tmax, T);
+    tmin = fmin(tmin, T);
+    double vxcen = (vx + a.f.vx[C + skp]) * 0.5;
+    double tcen = (T + a.f.temp[C + skp]) * 0.5;
+    an += tcen * vxcen * fac2;
+    tm += tcen * fac2;
+    x2 += fac2 * (vx * vx);
+    y2 += fac2 * (vy * vy);
+    z2 += fac2 * (vz * vz);
+    a2 += fac2 * (vz * vz + vy * vy + vx * vx);
+  }
+  double v[11] = {an, tm, x2, y2, z2, a2, mz, my, mx, tmax, tmin};
+  for (int q = 0; q < 11; q++) {
+    double r = q < 6 ? block_reduce(v[q], OpSum(), 0)
+                     : (q < 10 ? block_reduce(v[q], OpMax(), 0) : block_reduce(v[q], OpMin(), 0));
+    if (threadIdx.x == 0) a.partial[(int64_t)blockIdx.x * 11 + q] = r;
+  }
+}
+
+// CalcPlateNu.F90:38-44: sums over columns
+__global__ void k_red_platenu(ReduceArgs a, double del, double deln) {
+  const Geom g = a.g;
+  double lo = 0, up = 0;
+  const int s0 = pos_of_k(0, g.L), s1 = pos_of_k(1, g.L), sm = pos_of_k(g.nxm - 1, g.L),
+            sn = pos_of_k(g.nx - 1, g.L);
+  for (int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; col < g.ncol;
+       col += (int64_t)gridDim.x * blockDim.x) {
+    const double *T = a.f.temp + col * g.ldx;
+    lo += (T[s0] - T[s1]) * del;
+    up += (T[sm] - T[sn]) * deln;
+  }
+  double r0 = block_reduce(lo, OpSum(), 0), r1 = block_reduce(up, OpSum(), 0);
+  if (threadIdx.x == 0) {
+    a.partial[(int64_t)blockIdx.x * 2 + 0] = r0;
+    a.partial[(int64_t)blockIdx.x * 2 + 1] = r1;
+  }
+}
+
+// CalcDissipationNu.F90:63-113: one block per plane k; out[2+k] disste, out[2+nxm+k] dissth,
+// partial[k*2+{0,1}] = plane sums of nu_mu, nu_th contributions.
+__global__ void k_red_diss(ReduceArgs a) {
+  const Geom g = a.g;
+  const int k = blockIdx.x;
+  const int s = pos_of_k(k, g.L), sp = pos_of_k(k + 1, g.L);
+  double mu = 0, th = 0, pe = 0, pt = 0;
+  const double dxc = a.dxc[k], dxm = a.dxm[k], lem = a.lem[k], lec = a.lec[k];
+  for (int64_t col = threadIdx.x; col < g.ncol; col += blockDim.x) {
+    const int j = (int)(col % g.nym), i = (int)(col / g.nym);
+    const int64_t C = col * g.ldx, CJP = col_off(wrapp(j + 1, g.nym), i, g),
+                  CIP = col_off(j, wrapp(i + 1, g.nzm), g);
+    const double *vx = a.f.vx, *vy = a.f.vy, *vz = a.f.vz, *T = a.f.temp;
+    double hxx = (vx[C + sp] - vx[C + s]) * dxc;
+    double hxy = (vx[CJP + s] - vx[C + s]) * a.dy;
+    double hxz = (vx[CIP + s] - vx[C + s]) * a.dz;
+    double hyx = (vy[C + sp] - vy[C + s]) * dxm;
+    double hyy = (vy[CJP + s] - vy[C + s]) * a.dy;
+    double hyz = (vy[CIP + s] - vy[C + s]) * a.dz;
+    double hzx = (vz[C + sp] - vz[C + s]) * dxm;
+    double hzy = (vz[CJP + s] - vz[C + s]) * a.dy;
+    double hzz = (vz[CIP + s] - vz[C + s]) * a.dz;
+    double d1 = (hxx * hxx + hxy * hxy + hxz * hxz);
+    double d2 = (hyx * hyx + hyy * hyy + hyz * hyz) + (hzx * hzx + hzy * hzy + hzz * hzz);
+    mu += d1 * lem * a.pra + d2 * lec * a.pra;
+    double tx = (T[C + sp] - T[C + s]) * dxc;
+    double ty = (T[CJP + s] - T[C + s]) * a.dy;
+    double tz = (T[CIP + s] - T[C + s]) * a.dz;
+    double dth = tx * tx + ty * ty + tz * tz;
+    th += dth * lem;
+    pe += (d1 + d2);
+    pt += dth;
+  }
+  mu = block_reduce(mu, OpSum(), 0);
+  th = block_reduce(th, OpSum(), 0);
+  pe = block_reduce(pe, OpSum(), 0);
+  pt = block_reduce(pt, OpSum(), 0);
+  if (threadIdx.x == 0) {
+    const double nn = (double)g.nym * (double)g.nzm;
+    a.partial[k * 2 + 0] = mu;
+    a.partial[k * 2 + 1] = th;
+    a.out[2 + k] = pe / (a.ren * nn);
+    a.out[2 + g.nxm + k] = pt / (a.pec * nn);
+  }
+}
+
+static hipError_t finalize(const ReduceArgs &a, int nq, int maxmask, int minmask, hipStream_t st) {
+  hipLaunchKernelGGL(k_finalize, dim3(nq), dim3(256), 0, st, a.partial, a.out, nq, (const int *)nullptr,
+                     maxmask, minmask);
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce_cfl(const ReduceArgs &a, hipStream_t st) {
+  hipLaunchKernelGGL(k_red_cfl, dim3(RBLK), dim3(256), 0, st, a);
+  return finalize(a, 1, 1, 0, st);
+}
+hipError_t launch_reduce_divmax(const ReduceArgs &a, hipStream_t st) {
+  hipLaunchKernelGGL(k_red_div, dim3(RBLK), dim3(256), 0, st, a);
+  return finalize(a, 1, 1, 0, st);
+}
+hipError_t launch_reduce_global(const ReduceArgs &a, hipStream_t st) {
+  hipLaunchKernelGGL(k_red_global, dim3(RBLK), dim3(256), 0, st, a);
+  return finalize(a, 11, (0xF << 6), (1 << 10), st);
+}
+hipError_t launch_reduce_platenu(const ReduceArgs &a, double del, double deln, hipStream_t st) {
+  hipLaunchKernelGGL(k_red_platenu, dim3(RBLK), dim3(256), 0, st, a, del, deln);
+  return finalize(a, 2, 0, 0, st);
+}
+__global__ void k_diss_sum(ReduceArgs a) {
+  double mu = 0, th = 0;
+  for (int k = threadIdx.x; k < a.g.nxm; k += blockDim.x) {
+    mu += a.partial[k * 2];
+    th += a.partial[k * 2 + 1];
+  }
+  mu = block_reduce(mu, OpSum(), 0);
+  th = block_reduce(th, OpSum(), 0);
+  if (threadIdx.x == 0) {
+    a.out[0] = mu;
+    a.out[1] = th;
+  }
+}
+hipError_t launch_reduce_dissipation(const ReduceArgs &a, hipStream_t st) {
+  hipLaunchKernelGGL(k_red_diss, dim3(a.g.nxm), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_diss_sum, dim3(1), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- layout conversion
+// stage: raw copy of the host array; element (k,j,i) (0-based, k<n1) at
+// stage[k + ld1*((j+1-j0) + ld2*(i+1-i0))]  (j0/i0 are the host lower bounds).
+__global__ void k_gather_in(const double *stage, double *dst, Geom g, int n1, int ld1, int ld2, int j0,
+                            int i0) {
+  for (int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; gid < g.ncol * g.ldx;
+       gid += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t col = gid / g.ldx;
+    const int s = (int)(gid % g.ldx), k = k_of_pos(s, g.L);
+    if (k >= n1) continue;
+    const int j = (int)(col % g.nym), i = (int)(col / g.nym);
+    dst[gid] = stage[k + (int64_t)ld1 * ((j + 1 - j0) + (int64_t)ld2 * (i + 1 - i0))];
+  }
+}
+// inverse; with halo != 0 the host layout's halo planes (j = j0-1.. etc.) are
+// filled with periodic copies, as update_halo leaves them.
+__global__ void k_scatter_out(const double *src, double *stage, Geom g, int n1, int ld1, int ld2, int j0,
+                              int i0, int halo) {
+  const int ny = halo ? g.nym + 2 : g.nym, nz = halo ? g.nzm + 2 : g.nzm;
+  const int64_t total = (int64_t)n1 * ny * nz;
+  for (int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; gid < total;
+       gid += (int64_t)gridDim.x * blockDim.x) {
+    const int k = (int)(gid % n1);
+    const int64_t r = gid / n1;
+    const int jj = (int)(r % ny), ii = (int)(r / ny);
+    int j = halo ? jj - 1 : jj, i = halo ? ii - 1 : ii;  // 0-based interior index (may be -1 or n)
+    int jw = j < 0 ? j + g.nym : (j >= g.nym ? j - g.nym : j);
+    int iw = i < 0 ? i + g.nzm : (i >= g.nzm ? i - g.nzm : i);
+    stage[k + (int64_t)ld1 * ((j + 1 - j0) + (int64_t)ld2 * (i + 1 - i0))] =
+        src[col_off(jw, iw, g) + pos_of_k(k, g.L)];
+  }
+}
+hipError_t launch_gather_in(const double *stage, double *dst, const Geom &g, int n1, int ld1, int ld2, int j0,
+                            int i0, hipStream_t st) {
+  hipLaunchKernelGGL(k_gather_in, dim3(4096), dim3(256), 0, st, stage, dst, g, n1, ld1, ld2, j0, i0);
+  return hipGetLastError();
+}
+hipError_t launch_scatter_out(const double *src, double *stage, const Geom &g, int n1, int ld1, int ld2, int j0,
+                              int i0, int halo, hipStream_t st) {
+  hipLaunchKernelGGL(k_scatter_out, dim3(4096), dim3(256), 0, st, src, stage, g, n1, ld1, ld2, j0, i0, halo);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- initial condition
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+struct ZeroSet { double *p[4]; };
+__global__ void k_init_noise(double *temp, ZeroSet zs, const double *xc, Geom g, uint64_t seed, double amp,
+                             double tbp, double ttp) {
+  for (int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; gid < g.ncol * g.ldx;
+       gid += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t col = gid / g.ldx;
+    const int s = (int)(gid % g.ldx), k = k_of_pos(s, g.L);
+    for (int q = 0; q < 4; q++) zs.p[q][gid] = 0.0;
+    if (k >= g.nx) {
+      temp[gid] = 0.0;
+      continue;
+    }
+    const int j = (int)(col % g.nym), i = (int)(col / g.nym);
+    double v;
+    if (k == 0) v = tbp;
+    else if (k == g.nx - 1) v = ttp;
+    else {
+      uint64_t lin = (uint64_t)k + (uint64_t)g.nx * ((uint64_t)j + (uint64_t)g.nym * (uint64_t)i);
+      uint64_t h = mix64(seed * 0x9E3779B97F4A7C15ULL + (lin + 1) * 0xD1B54A32D192ED03ULL);
+      double u = (double)(h >> 11) * 0x1.0p-53;
+      v = 1.0 - xc[k] + (2.0 * u - 1.0) * amp;
+    }
+    temp[gid] = v;
+  }
+}
+hipError_t launch_init_noise(double *temp, double *zero_fields[4], const double *xc_nat, const Geom &g,
+                             uint64_t seed, double amp, double tbp, double ttp, hipStream_t st) {
+  ZeroSet zs;
+  for (int q = 0; q < 4; q++) zs.p[q] = zero_fields[q];
+  hipLaunchKernelGGL(k_init_noise, dim3(4096), dim3(256), 0, st, temp, zs, xc_nat, g, seed, amp, tbp, ttp);
+  return hipGetLastError();
+}
+
+__global__ void k_fill(double *p, double v, int64_t n) {
+  for (int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; gid < n; gid += (int64_t)gridDim.x * blockDim.x)
+    p[gid] = v;
+}
+hipError_t launch_fill(double *p, double v, int64_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill, dim3(2048), dim3(256), 0, st, p, v, n);
+  return hipGetLastError();
+}
+
+}  // namespace afid

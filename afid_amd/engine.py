@@ -1,0 +1,304 @@
+"""Host-side mirror of AFiD's substep interface over the C ABI (include/afid.h).
+
+The reference exposes argument-less Fortran subroutines acting on module
+globals (param.F90 / local_arrays).  Here a ``Cell`` owns one device context
+(the role of those modules) and offers the same routine names:
+
+    ExplicitTermsVX/VY/VZ/Temp, ImplicitAndUpdateVX/VY/VZ/Temp,
+    CalcLocalDivergence, SolvePressureCorrection, CorrectVelocity,
+    CorrectPressure, update_halo, TimeMarcher,
+    CalcMaxCFL, CheckDivergence, CalcPlateNu, GlobalQuantities, CalcDissipationNu
+
+Arrays cross the boundary in the reference's Fortran layout (x fastest);
+numpy views here are indexed [i, j, k] (z, y, x).  The library is loaded from
+afid_amd/lib/libafid_hip.so; there is no fallback: if it is missing or a HIP
+call fails, an AfidError is raised.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libafid_hip.so")
+
+AFID_VX, AFID_VY, AFID_VZ, AFID_TEMP, AFID_PR = range(5)
+AFID_RUX, AFID_RUY, AFID_RUZ, AFID_RUTEMP, AFID_DPH, AFID_QCAP, AFID_DQ, AFID_HRO, AFID_RHS = range(5, 14)
+FIELD_IDS = {"vx": AFID_VX, "vy": AFID_VY, "vz": AFID_VZ, "temp": AFID_TEMP, "pr": AFID_PR,
+             "rux": AFID_RUX, "ruy": AFID_RUY, "ruz": AFID_RUZ, "rutemp": AFID_RUTEMP, "dph": AFID_DPH,
+             "qcap": AFID_QCAP, "dq": AFID_DQ, "hro": AFID_HRO, "rhs": AFID_RHS}
+
+
+class AfidError(RuntimeError):
+    pass
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [("nxm", ctypes.c_int), ("nym", ctypes.c_int), ("nzm", ctypes.c_int),
+                ("nsst", ctypes.c_int), ("istr3", ctypes.c_int), ("str3", ctypes.c_double),
+                ("alx3", ctypes.c_double), ("ylen", ctypes.c_double), ("zlen", ctypes.c_double),
+                ("ray", ctypes.c_double), ("pra", ctypes.c_double), ("inslws", ctypes.c_int),
+                ("inslwn", ctypes.c_int), ("tempbp", ctypes.c_double), ("temptp", ctypes.c_double)]
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load libafid_hip.so; raises AfidError if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise AfidError(f"{path} not found: build it with afid_amd.build.build() (hipcc, gfx950)")
+    L = ctypes.CDLL(path)
+    vp, i, d, i64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int64
+    pd = ctypes.POINTER(d)
+    sig = {
+        "afid_create": ([ctypes.POINTER(vp), ctypes.POINTER(_Params), vp], i),
+        "afid_destroy": ([vp], i),
+        "afid_strerror": ([i], ctypes.c_char_p),
+        "afid_last_hip_error": ([vp], i),
+        "afid_get_dims": ([vp, ctypes.POINTER(i64)], i),
+        "afid_get_metric": ([vp, ctypes.c_char_p, pd, i], i),
+        "afid_upload": ([vp, i, pd, i, i, i, i], i),
+        "afid_download": ([vp, i, pd, i, i, i, i], i),
+        "afid_init_linear_noise": ([vp, ctypes.c_uint64, d], i),
+        "afid_set_coefficients": ([vp, d, d, d, d, d], i),
+        "afid_set_substep": ([vp, i, d], i),
+        "afid_substep": ([vp, i, d], i),
+        "afid_time_marcher": ([vp, d], i),
+        "afid_step": ([vp, i, i, d, d, d, d, pd, ctypes.POINTER(i)], i),
+        "afid_get_time": ([vp, pd], i),
+        "afid_calc_max_cfl": ([vp, pd], i),
+        "afid_check_divergence": ([vp, pd], i),
+        "afid_calc_plate_nu": ([vp, pd, pd], i),
+        "afid_global_quantities": ([vp, pd], i),
+        "afid_calc_dissipation_nu": ([vp, pd, pd, pd, pd], i),
+        "afid_update_halo": ([vp, i], i),
+        "afid_pressure_spectrum": ([vp, i, pd], i),
+        "afid_profile_enable": ([vp, i], i),
+        "afid_profile_read": ([vp, ctypes.c_char_p, i, pd, ctypes.POINTER(i64), i], i),
+        "afid_profile_reset": ([vp], i),
+        "afid_synchronize": ([vp], i),
+    }
+    for name in ("explicit_terms_vx", "explicit_terms_vy", "explicit_terms_vz", "explicit_terms_temp",
+                 "implicit_and_update_vx", "implicit_and_update_vy", "implicit_and_update_vz",
+                 "implicit_and_update_temp", "calc_local_divergence", "solve_pressure_correction",
+                 "correct_velocity", "correct_pressure"):
+        sig["afid_" + name] = ([vp], i)
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def exported_symbols() -> list[str]:
+    """Entry points declared in include/afid.h (for the export check)."""
+    hdr = os.path.join(os.path.dirname(HERE), "include", "afid.h")
+    import re
+    names = re.findall(r"^(?:int|const char \*)\s*(afid_\w+)\s*\(", open(hdr).read(), flags=re.M)
+    return names
+
+
+@dataclass
+class CellParams:
+    """bou.in subset (ReadInputFile.F90:16-36)."""
+    nxm: int
+    nym: int
+    nzm: int
+    nsst: int = 3
+    istr3: int = 0
+    str3: float = 0.0
+    alx3: float = 1.0
+    ylen: float = 1.0
+    zlen: float = 1.0
+    ray: float = 1e6
+    pra: float = 1.0
+    inslws: int = 1
+    inslwn: int = 1
+    tempbp: float = 1.0
+    temptp: float = 0.0
+
+    def to_c(self) -> _Params:
+        return _Params(self.nxm, self.nym, self.nzm, self.nsst, self.istr3, self.str3, self.alx3,
+                       self.ylen, self.zlen, self.ray, self.pra, self.inslws, self.inslwn,
+                       self.tempbp, self.temptp)
+
+    @property
+    def cells(self) -> int:
+        return self.nxm * self.nym * self.nzm
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Cell:
+    """One Rayleigh-Benard cell resident on the GPU."""
+
+    def __init__(self, params: CellParams):
+        self.L = load_library()
+        self.params = params
+        self.h = ctypes.c_void_p()
+        cp = params.to_c()
+        self._check(self.L.afid_create(ctypes.byref(self.h), ctypes.byref(cp), None), "afid_create")
+        dims = (ctypes.c_int64 * 6)()
+        self._check(self.L.afid_get_dims(self.h, dims), "afid_get_dims")
+        self.nx, self.nxm, self.nym, self.nzm, self.ldx, self.device_bytes = list(dims)
+
+    # ---- plumbing ---------------------------------------------------------------
+    def _check(self, rc: int, what: str):
+        if rc != 0:
+            msg = self.L.afid_strerror(rc).decode()
+            hip = self.L.afid_last_hip_error(self.h) if self.h else 0
+            raise AfidError(f"{what}: {msg} (code {rc}, hip {hip})")
+
+    def close(self):
+        if self.h:
+            self.L.afid_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def metric(self, name: str, n: int) -> np.ndarray:
+        out = np.zeros(n)
+        self._check(self.L.afid_get_metric(self.h, name.encode(), _dp(out), n), f"metric {name}")
+        return out
+
+    # ---- data exchange ----------------------------------------------------------
+    def upload(self, name: str, a: np.ndarray, halo: bool = False):
+        """a: (nzm[+2], nym[+2], n1) C-order = Fortran (n1, nym, nzm) with x fastest."""
+        fid = FIELD_IDS[name]
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        j0 = i0 = 0 if halo else 1
+        self._check(self.L.afid_upload(self.h, fid, _dp(a), a.shape[2], a.shape[1], j0, i0), f"upload {name}")
+
+    def download(self, name: str, halo: bool = False) -> np.ndarray:
+        fid = FIELD_IDS[name]
+        n1 = self.nxm if name == "dph" else self.nx
+        ny, nz = (self.nym + 2, self.nzm + 2) if halo else (self.nym, self.nzm)
+        a = np.zeros((nz, ny, n1))
+        j0 = i0 = 0 if halo else 1
+        self._check(self.L.afid_download(self.h, fid, _dp(a), n1, ny, j0, i0), f"download {name}")
+        return a
+
+    def init_linear_noise(self, seed: int, amp: float = 1e-3):
+        self._check(self.L.afid_init_linear_noise(self.h, seed, amp), "init_linear_noise")
+
+    def synchronize(self):
+        self._check(self.L.afid_synchronize(self.h), "synchronize")
+
+    # ---- reference routine names -------------------------------------------------
+    def set_coefficients(self, al, ga, ro, dt, beta):
+        self._check(self.L.afid_set_coefficients(self.h, al, ga, ro, dt, beta), "set_coefficients")
+
+    def set_substep(self, ns: int, dt: float):
+        self._check(self.L.afid_set_substep(self.h, ns, dt), "set_substep")
+
+    def _call(self, name):
+        self._check(getattr(self.L, "afid_" + name)(self.h), name)
+
+    def ExplicitTermsVX(self): self._call("explicit_terms_vx")
+    def ExplicitTermsVY(self): self._call("explicit_terms_vy")
+    def ExplicitTermsVZ(self): self._call("explicit_terms_vz")
+    def ExplicitTermsTemp(self): self._call("explicit_terms_temp")
+    def ImplicitAndUpdateVX(self): self._call("implicit_and_update_vx")
+    def ImplicitAndUpdateVY(self): self._call("implicit_and_update_vy")
+    def ImplicitAndUpdateVZ(self): self._call("implicit_and_update_vz")
+    def ImplicitAndUpdateTemp(self): self._call("implicit_and_update_temp")
+    def CalcLocalDivergence(self): self._call("calc_local_divergence")
+    def SolvePressureCorrection(self): self._call("solve_pressure_correction")
+    def CorrectVelocity(self): self._call("correct_velocity")
+    def CorrectPressure(self): self._call("correct_pressure")
+
+    def pressure_spectrum(self, stage: int) -> np.ndarray:
+        """Spectrum after `stage` steps of SolvePressureCorrection, indexed [n, m, k]."""
+        nymh = self.nym // 2 + 1
+        out = np.zeros((self.nzm, nymh, self.nxm), dtype=np.complex128)
+        self._check(self.L.afid_pressure_spectrum(self.h, stage, _dp(out.view(np.float64))), "pressure_spectrum")
+        return out
+
+    def update_halo(self, name: str):
+        self._check(self.L.afid_update_halo(self.h, FIELD_IDS[name]), "update_halo")
+
+    def substep(self, ns: int, dt: float):
+        self._check(self.L.afid_substep(self.h, ns, dt), "substep")
+
+    def TimeMarcher(self, dt: float):
+        self._check(self.L.afid_time_marcher(self.h, dt), "TimeMarcher")
+
+    def step(self, ntime: int, idtv: int, dt: float, dtmin: float = 1e-8, dtmax: float = 1e-2,
+             limitcfl: float = 1.2):
+        used = ctypes.c_double()
+        err = ctypes.c_int()
+        self._check(self.L.afid_step(self.h, ntime, idtv, dt, dtmin, dtmax, limitcfl, ctypes.byref(used),
+                                     ctypes.byref(err)), "step")
+        return err.value, used.value
+
+    @property
+    def time(self) -> float:
+        t = ctypes.c_double()
+        self._check(self.L.afid_get_time(self.h, ctypes.byref(t)), "get_time")
+        return t.value
+
+    # ---- diagnostics -------------------------------------------------------------
+    def CalcMaxCFL(self) -> float:
+        v = ctypes.c_double()
+        self._check(self.L.afid_calc_max_cfl(self.h, ctypes.byref(v)), "CalcMaxCFL")
+        return v.value
+
+    def CheckDivergence(self) -> float:
+        v = ctypes.c_double()
+        self._check(self.L.afid_check_divergence(self.h, ctypes.byref(v)), "CheckDivergence")
+        return v.value
+
+    def CalcPlateNu(self):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        self._check(self.L.afid_calc_plate_nu(self.h, ctypes.byref(a), ctypes.byref(b)), "CalcPlateNu")
+        return a.value, b.value
+
+    def GlobalQuantities(self) -> np.ndarray:
+        out = np.zeros(11)
+        self._check(self.L.afid_global_quantities(self.h, _dp(out)), "GlobalQuantities")
+        return out
+
+    def CalcDissipationNu(self):
+        mu, th = ctypes.c_double(), ctypes.c_double()
+        de, dt_ = np.zeros(self.nxm), np.zeros(self.nxm)
+        self._check(self.L.afid_calc_dissipation_nu(self.h, ctypes.byref(mu), ctypes.byref(th), _dp(de),
+                                                    _dp(dt_)), "CalcDissipationNu")
+        return mu.value, th.value, de, dt_
+
+    # ---- profiling ----------------------------------------------------------------
+    def profile(self, on: bool = True):
+        self._check(self.L.afid_profile_enable(self.h, 1 if on else 0), "profile_enable")
+
+    def profile_reset(self):
+        self._check(self.L.afid_profile_reset(self.h), "profile_reset")
+
+    def profile_read(self) -> dict:
+        cap = 64
+        buf = ctypes.create_string_buffer(4096)
+        ms = (ctypes.c_double * cap)()
+        cnt = (ctypes.c_int64 * cap)()
+        n = self.L.afid_profile_read(self.h, buf, 4096, ms, cnt, cap)
+        if n < 0:
+            self._check(n, "profile_read")
+        names = buf.raw.split(b"\0")[:n]
+        return {nm.decode(): (ms[q], cnt[q]) for q, nm in enumerate(names)}

@@ -1,0 +1,141 @@
+"""Throughput of AFiD's time advance on one MI355X.
+
+Workload (default): production single cell, nxm=384 (clipped-Chebyshev
+stretching istr3=6, str3=6), ny=nz=768 periodic, ylen=zlen=1, Ra=1e8, Pr=1,
+RK3, CFL-limited dt (cflmax 1.2, dtmin 1e-8, dtmax 1e-2), seeded conduction
+profile plus +-1e-3 noise, FP64.  One step = one main.F90 time step: CalcMaxCFL,
+dt selection, three fused substeps.  Inputs are resident in HBM before timing.
+
+Prints one JSON line (metric Mcell-updates/s = nxm*nym*nzm*steps/s/1e6) with a
+roofline block for the dominant kernel (HIP events on the engine's stream) and a
+CPU baseline from the oracle on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic FP64 words per cell (compulsory traffic of the reference routines):
+# explicit VX 5, VY 4, VZ 4, Temp 5; implicit VX/VY/VZ 6 each, Temp 5;
+# divergence 4; pressure solve 10 (5 passes); CorrectVelocity 7; CorrectPressure 3.
+WORDS = {
+    "fused_update": 5 + 4 + 4 + 5 + 6 + 6 + 6 + 5,
+    "div_yfft_r2c": 4 + 2,
+    "zfft_fwd": 2,
+    "xsolve_pressure": 2,
+    "zfft_bwd": 2,
+    "yfft_c2r": 2,
+    "correct_vel_pr": 7 + 3,
+}
+SUBSTEP_WORDS = 65
+
+CONFIGS = {
+    "cfg2": dict(nxm=384, nym=768, nzm=768, istr3=6, str3=6.0, ylen=1.0, zlen=1.0, ray=1e8, pra=1.0, seed=2),
+    "cfg1": dict(nxm=64, nym=128, nzm=128, istr3=6, str3=6.0, ylen=2.0, zlen=2.0, ray=1e6, pra=1.0, seed=1),
+}
+
+
+def cpu_baseline(cfg: dict, threads: int) -> dict:
+    """Oracle (C restatement, OpenMP) on one RK3 substep of the same cell."""
+    from oracle import oracle as O
+    O.lib().afo_set_threads(threads)
+    p = {k: v for k, v in cfg.items() if k != "seed"}
+    o = O.Oracle(nsst=3, alx3=1.0, **p)
+    o.init_linear_noise(cfg["seed"], 1e-3)
+    t0 = time.perf_counter()
+    o.substep(1, 1e-4)
+    t = time.perf_counter() - t0
+    cells = p["nxm"] * p["nym"] * p["nzm"]
+    o.close()
+    return {"value": round(cells / (3 * t) / 1e6, 3), "unit": "Mcell-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"one RK3 substep (of 3 per step) of the same {p['nxm']}x{p['nym']}x{p['nzm']} cell, "
+                      f"{t:.1f} s on {threads} OpenMP threads; rate scaled to full RK3 steps"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="cfg2", choices=sorted(CONFIGS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
+    ap.add_argument("--profile-steps", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(each cell runs on one device; see DESIGN.md for how the path would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    from afid_amd import Cell, CellParams
+    cfg = CONFIGS[args.config]
+    p = {k: v for k, v in cfg.items() if k != "seed"}
+    cell = Cell(CellParams(nsst=3, alx3=1.0, **p))
+    cell.init_linear_noise(cfg["seed"], 1e-3)
+    cells = p["nxm"] * p["nym"] * p["nzm"]
+    ctl = dict(idtv=1, dt=1e-3, dtmin=1e-8, dtmax=1e-2, limitcfl=1.2)
+    ntime = 0
+    for _ in range(args.warmup):
+        cell.step(ntime, ctl["idtv"], ctl["dt"], ctl["dtmin"], ctl["dtmax"], ctl["limitcfl"])
+        ntime += 1
+    cell.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        err, dt = cell.step(ntime, ctl["idtv"], ctl["dt"], ctl["dtmin"], ctl["dtmax"], ctl["limitcfl"])
+        ntime += 1
+    cell.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = elapsed / args.steps * 1e3
+    value = cells * args.steps / elapsed / 1e6
+    divmax = cell.CheckDivergence()
+
+    # per-kernel HIP-event timing on the engine stream (separate short pass)
+    cell.profile(True)
+    cell.profile_reset()
+    for _ in range(args.profile_steps):
+        cell.step(ntime, ctl["idtv"], ctl["dt"], ctl["dtmin"], ctl["dtmax"], ctl["limitcfl"])
+        ntime += 1
+    prof = cell.profile_read()
+    cell.profile(False)
+    kern = {k: {"avg_ms": v[0] / v[1], "launches": v[1]} for k, v in prof.items() if v[1] > 0}
+    substep_ms = sum(v[0] for k, v in prof.items() if k in WORDS) / (3 * args.profile_steps)
+    dom = max((k for k in kern if k in WORDS), key=lambda k: kern[k]["avg_ms"] * kern[k]["launches"])
+    alg_bytes = WORDS[dom] * 8 * cells
+    achieved = alg_bytes / (kern[dom]["avg_ms"] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "alg_bytes_per_launch": alg_bytes, "avg_ms": round(kern[dom]["avg_ms"], 4)}
+    substep_alg = SUBSTEP_WORDS * 8 * cells / (substep_ms * 1e-3) / 1e9
+    cell.close()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(cfg, args.cpu_threads)
+
+    out = {
+        "metric": "Mcell-updates/s", "value": round(value, 2), "unit": "Mcell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded conduction profile + 1e-3 noise)",
+        "config": {"workload": f"{args.config}: RB cell {p['nxm']}x{p['nym']}x{p['nzm']} istr3={p['istr3']} "
+                               f"str3={p['str3']} Ra={p['ray']:g} Pr={p['pra']:g} RK3 CFL dt", "cells": cells},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "substep": {"ms": round(substep_ms, 3), "alg_GBps_65words": round(substep_alg, 1),
+                    "frac_hbm_65words": round(substep_alg / HBM_PEAK_GBS, 4)},
+        "kernels_ms": {k: round(v["avg_ms"], 4) for k, v in kern.items()},
+        "max_divergence": divmax,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

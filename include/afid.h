@@ -1,0 +1,160 @@
+/*
+ * afid.h -- C ABI of the MI355X engine for AFiD's per-substep time advance.
+ *
+ * Plain C types only (pointers, sizes, doubles).  One opaque context owns all
+ * device buffers, streams and FFT tables for one Rayleigh-Benard cell; host
+ * arrays are borrowed only for the duration of a synchronous upload/download.
+ *
+ * Each entry point replaces one reference routine (AFiD 1.1, Fortran).  The
+ * reference routines take no arguments and work on module globals (param.F90,
+ * local_arrays); here the context plays the role of those modules.  The
+ * Fortran binding (iso_c_binding) lives in afid_amd/fortran/afid_hip.F90.
+ *
+ * Return codes: 0 = AFID_OK; < 0 = runtime/HIP/argument error; > 0 =
+ * numerical condition (see afid_strerror).  Calls on one context are not
+ * re-entrant; they are stream-ordered on the context's stream, and every
+ * call that returns data to the host synchronises that stream first.
+ */
+#ifndef AFID_H
+#define AFID_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AFID_OK 0
+#define AFID_ERR_ARG (-1)        /* invalid argument or size                 */
+#define AFID_ERR_HIP (-2)        /* HIP runtime error (see afid_last_hip_error) */
+#define AFID_ERR_NOMEM (-3)      /* device allocation failed                 */
+#define AFID_ERR_FFTSIZE (-4)    /* lateral size has a prime factor > 5 (errorcode 444) */
+#define AFID_ERR_STATE (-5)      /* call not valid in the context's state    */
+#define AFID_NUM_PIVOT 1         /* an implicit factorisation would need a row
+                                    interchange (dgttrf pivoting) */
+#define AFID_NUM_SINGULAR 2      /* singular pressure tridiagonal (zgttrf info>0),
+                                    SolvePressureCorrection.F90:130-134 */
+
+typedef struct afid_ctx afid_ctx;
+
+/* bou.in subset, ReadInputFile.F90:16-36 (names as in param.F90:8-19). */
+typedef struct afid_params {
+  int nxm, nym, nzm;   /* cells: wall-normal x, periodic y, periodic z        */
+  int nsst;            /* >1: RK3 (3 substeps), else AB2 (InitTimeMarchScheme) */
+  int istr3;           /* x clustering: 0 uniform, 4 tanh, 6 clipped Chebyshev */
+  double str3;         /* clustering strength                                  */
+  double alx3;         /* cell height                                          */
+  double ylen, zlen;   /* periodic extents                                     */
+  double ray, pra;     /* Rayleigh, Prandtl                                    */
+  int inslws, inslwn;  /* 1 no-slip, 0 stress-free: lower / upper plate        */
+  double tempbp, temptp; /* plate temperatures, SetTempBCs.F90:15-20 (1, 0)   */
+} afid_params;
+
+/* Optional caller-computed 1-D metrics, Fortran 1-based arrays as declared in
+ * param.F90:24-48 (pass the address of element 1).  Any NULL member is
+ * computed by the library with CreateGrid.F90 / InitPressureSolver.F90. */
+typedef struct afid_metrics {
+  const double *xc, *xm, *g3rc, *g3rm, *udx3c, *udx3m;      /* length nx  */
+  const double *ap3ck, *ac3ck, *am3ck, *ap3sk, *ac3sk, *am3sk;
+  const double *ap3ssk, *ac3ssk, *am3ssk;                    /* length nx  */
+  const double *amphk, *acphk, *apphk;                       /* length nxm */
+  const double *ak1, *ak2;                                   /* nzm, nym   */
+} afid_metrics;
+
+/* Field identifiers for upload/download (local_arrays, param.F90:79-87). */
+enum afid_field {
+  AFID_VX = 0, AFID_VY, AFID_VZ, AFID_TEMP, AFID_PR,
+  AFID_RUX, AFID_RUY, AFID_RUZ, AFID_RUTEMP,
+  AFID_DPH,   /* dph(1:nxm,..): ExplicitTermsVY output, then pressure correction */
+  AFID_QCAP, AFID_DQ, AFID_HRO, AFID_RHS,
+  AFID_NFIELDS
+};
+
+/* ---- lifetime (InitVariables.F90, CreateGrid.F90, InitPressureSolver.F90,
+ *      InitTimeMarchScheme.F90, SetTempBCs.F90) --------------------------- */
+int afid_create(afid_ctx **ctx, const afid_params *p, const afid_metrics *m);
+int afid_destroy(afid_ctx *ctx);
+const char *afid_strerror(int code);
+int afid_last_hip_error(afid_ctx *ctx);
+/* Sizes: out[0..]= nx, nxm, nym, nzm, device x pitch, bytes of device memory held. */
+int afid_get_dims(afid_ctx *ctx, int64_t *out6);
+/* 1-D metric copies (1-based name as in param.F90), n values into out. */
+int afid_get_metric(afid_ctx *ctx, const char *name, double *out, int n);
+
+/* ---- host <-> device (replaces the module arrays as the data exchange) ----
+ * Host element (k,j,i), k in 1..n1 (n1 = nx, or nxm for AFID_DPH),
+ * j in 1..nym, i in 1..nzm, is host[(k-1) + ld1*((j-j0) + ld2*(i-i0))]:
+ * a reference halo field (1:nx,0:nym+1,0:nzm+1) is ld1=nx, ld2=nym+2,
+ * j0=i0=0; a plain field (1:nx,1:nym,1:nzm) is ld1=nx, ld2=nym, j0=i0=1.
+ * Downloads into a halo layout also fill the periodic halos (update_halo). */
+int afid_upload(afid_ctx *ctx, int field, const double *host, int ld1, int ld2, int j0, int i0);
+int afid_download(afid_ctx *ctx, int field, double *host, int ld1, int ld2, int j0, int i0);
+/* Seeded initial state: vx=vy=vz=pr=0, temp(1)=tempbp, temp(nx)=temptp,
+ * temp(k)=1-xc(k)+(2u-1)*amp, u from splitmix64(seed, linear index). */
+int afid_init_linear_noise(afid_ctx *ctx, uint64_t seed, double amp);
+
+/* ---- per-routine entry points (one reference subroutine each) ------------
+ * Substep coefficients as TimeMarcher.F90:22-30 sets them: al, ga, ro, dt and
+ * beta = dt/ren/2.  afid_set_substep(ctx, ns, dt) takes them from the scheme. */
+int afid_set_coefficients(afid_ctx *ctx, double al, double ga, double ro, double dt, double beta);
+int afid_set_substep(afid_ctx *ctx, int ns, double dt);
+int afid_explicit_terms_vx(afid_ctx *ctx);        /* ExplicitTermsVX.F90:11-109   -> qcap */
+int afid_explicit_terms_vy(afid_ctx *ctx);        /* ExplicitTermsVY.F90:11-105   -> dph  */
+int afid_explicit_terms_vz(afid_ctx *ctx);        /* ExplicitTermsVZ.F90:11-109   -> dq   */
+int afid_explicit_terms_temp(afid_ctx *ctx);      /* ExplicitTermsTemp.F90:11-100 -> hro  */
+int afid_implicit_and_update_vx(afid_ctx *ctx);   /* ImplicitAndUpdateVX.F90:14-80 + SolveImpEqnUpdate_X.F90 */
+int afid_implicit_and_update_vy(afid_ctx *ctx);   /* ImplicitAndUpdateVY.F90:14-78 + SolveImpEqnUpdate_YZ.F90 */
+int afid_implicit_and_update_vz(afid_ctx *ctx);   /* ImplicitAndUpdateVZ.F90:14-75 + SolveImpEqnUpdate_YZ.F90 */
+int afid_implicit_and_update_temp(afid_ctx *ctx); /* ImplicitAndUpdateTemp.F90:13-77 + SolveImpEqnUpdate_Temp.F90 */
+int afid_calc_local_divergence(afid_ctx *ctx);    /* CalcLocalDivergence.F90:12-44 -> dph */
+int afid_solve_pressure_correction(afid_ctx *ctx);/* SolvePressureCorrection.F90:12-154, dph in place */
+int afid_correct_velocity(afid_ctx *ctx);         /* CorrectVelocity.F90:11-48 (uses dph) */
+int afid_correct_pressure(afid_ctx *ctx);         /* CorrectPressure.F90:11-55 (uses dph) */
+int afid_update_halo(afid_ctx *ctx, int field);   /* halo.F90 update_halo: no-op, the
+                                                     device applies periodic wrap */
+/* Stages of SolvePressureCorrection on the current dph, for inspection: runs
+ * stage 1 (y r2c FFT, :98), 2 (+ z forward FFT, :102), 3 (+ x tridiagonal
+ * solves incl. the 1/(nym nzm) scaling, :105-141) or 4 (+ z backward FFT,
+ * :145) and copies the spectrum cz1(k, m, n) into out as interleaved complex,
+ * Fortran order (nxm, nym/2+1, nzm).  dph itself is left unchanged. */
+int afid_pressure_spectrum(afid_ctx *ctx, int stage, double *out);
+
+/* ---- fused time advance (TimeMarcher.F90:13-78) -----------------------------
+ * One substep / one full TimeMarcher call with fused kernels: explicit terms,
+ * implicit solves and updates in one pass; divergence + y-FFT; z-FFT; x
+ * tridiagonals; inverse FFTs; corrections.  qcap/dq/hro/rhs are not kept. */
+int afid_substep(afid_ctx *ctx, int ns, double dt);
+int afid_time_marcher(afid_ctx *ctx, double dt);
+
+/* main.F90:169-200 step body: CalcMaxCFL, dt selection (idtv=1: CFL-limited,
+ * dtmin for ntime<=1, clipped at dtmax; idtv=0: fixed dt), TimeMarcher,
+ * time += dt.  *errorcode receives 0, 165 (CFL too high) or 166 (dt<dtmin). */
+int afid_step(afid_ctx *ctx, int ntime, int idtv, double dt, double dtmin, double dtmax,
+              double limitcfl, double *dt_used, int *errorcode);
+int afid_get_time(afid_ctx *ctx, double *time);
+
+/* ---- diagnostics (FP64 device reductions) ----------------------------------- */
+int afid_calc_max_cfl(afid_ctx *ctx, double *cflm);          /* CalcMaxCFL.F90:11-48 */
+int afid_check_divergence(afid_ctx *ctx, double *qmax);      /* CheckDivergence.F90:11-48 */
+int afid_calc_plate_nu(afid_ctx *ctx, double *nu_bot, double *nu_top); /* CalcPlateNu.F90:11-57 */
+/* GlobalQuantities.F90:11-102: out[0]=Nu_vol, [1]=tempm, [2]=tempmax, [3]=tempmin,
+ * [4..6]=vmax(1..3) (vz,vy,vx), [7]=vz_rms, [8]=vy_rms, [9]=vx_rms, [10]=vzvyvx_rms */
+int afid_global_quantities(afid_ctx *ctx, double *out11);
+/* CalcDissipationNu.F90:12-120: Nu_eps_u, Nu_eps_T and this call's per-plane
+ * contributions disste/dissth (nxm values each; either may be NULL). */
+int afid_calc_dissipation_nu(afid_ctx *ctx, double *nu_mu, double *nu_th, double *disste,
+                             double *dissth);
+
+/* ---- profiling ----------------------------------------------------------------
+ * When enabled, every kernel launch is bracketed by HIP events on the
+ * context's stream; totals are per kernel name. */
+int afid_profile_enable(afid_ctx *ctx, int on);
+/* Writes up to cap entries: names (NUL-separated into names_buf of size nbuf),
+ * total milliseconds and launch counts.  Returns the number of kernels. */
+int afid_profile_read(afid_ctx *ctx, char *names_buf, int nbuf, double *total_ms, int64_t *count,
+                      int cap);
+int afid_profile_reset(afid_ctx *ctx);
+int afid_synchronize(afid_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
