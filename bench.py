@@ -60,8 +60,28 @@ def cpu_baseline(cfg: dict, threads: int) -> dict:
                       f"{t:.1f} s on {threads} OpenMP threads; rate scaled to full RK3 steps"}
 
 
+def copy_calibration(mib: int) -> None:
+    """One device-to-device copy of `mib` MiB (read + write of known size), for
+    scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings of the engine's kernels."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(n)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, ctypes.c_size_t(n))
+    hip.hipDeviceSynchronize()
+    hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="run one device-to-device copy of MIB MiB first (PMC traffic calibration)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
@@ -75,6 +95,8 @@ def main():
               "(each cell runs on one device; see DESIGN.md for how the path would shard)", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
     from afid_amd import Cell, CellParams
     cfg = CONFIGS[args.config]
     p = {k: v for k, v in cfg.items() if k != "seed"}
