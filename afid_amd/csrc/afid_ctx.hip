@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -46,6 +47,8 @@ struct afid_ctx {
   // device state
   double *f[AFID_NFIELDS] = {nullptr};
   double *fn[4] = {nullptr};   // second buffers of vx, vy, vz, temp for the fused update
+  double *bp[4] = {nullptr};   // split update: right-hand sides b + A q_old of vx, vy, vz, temp
+  int split = 1;               // 1: stencil pass + column-solve pass, 0: one fused pass
   double *stale = nullptr;     // rhs(1,j,i)
   double *kc = nullptr;
   double *fac_x = nullptr;     // per-routine factors (SYS_N*FR_N*ldx), current coefficients
@@ -310,6 +313,11 @@ int factor_system(const Geom &g, int n, const std::vector<double> &dl, const std
   interleave(g, e.data(), n, rows + FR_E * g.ldx);
   interleave(g, G.data(), n, rows + FR_G * g.ldx);
   interleave(g, H.data(), n, rows + FR_H * g.ldx);
+  std::vector<double> dlk(n, 0.0), duk(n, 0.0);
+  for (int k = 1; k < n; k++) dlk[k] = dl[k];
+  for (int k = 0; k + 1 < n; k++) duk[k] = du[k];
+  interleave(g, dlk.data(), n, rows + FR_DL * g.ldx);
+  interleave(g, duk.data(), n, rows + FR_DU * g.ldx);
   return rc;
 }
 
@@ -573,6 +581,7 @@ int afid_destroy(afid_ctx *c) {
   }
   for (int q = 0; q < AFID_NFIELDS; q++) hipFree(c->f[q]);
   for (int q = 0; q < 4; q++) hipFree(c->fn[q]);
+  for (int q = 0; q < 4; q++) hipFree(c->bp[q]);
   for (int q = 0; q < 4; q++) hipFree(c->fac_sub[q]);
   void *ps[] = {c->stale, c->kc, c->fac_x, c->S, c->cp, c->nat, c->partial, c->out, c->stage,
                 c->dpy.tw, c->dpy.rev, c->dpy.post, c->dpz.tw, c->dpz.rev, c->dpz.post};
@@ -652,6 +661,9 @@ int afid_create(afid_ctx **out, const afid_params *p, const afid_metrics *m) {
   for (int q : {AFID_VX, AFID_VY, AFID_VZ, AFID_TEMP, AFID_PR, AFID_RUX, AFID_RUY, AFID_RUZ, AFID_RUTEMP, AFID_DPH})
     TRY(alloc_field(c, &c->f[q]));
   for (int q = 0; q < 4; q++) TRY(alloc_field(c, &c->fn[q]));
+  if (const char *e = getenv("AFID_UPDATE")) c->split = strcmp(e, "fused") != 0;
+  if (c->split)
+    for (int q = 0; q < 4; q++) TRY(alloc_field(c, &c->bp[q]));
   TRYH(hipMalloc(&c->stale, sizeof(double) * g.ncol));
   TRYH(hipMemsetAsync(c->stale, 0, sizeof(double) * g.ncol, c->st));
   {
@@ -994,8 +1006,16 @@ int afid_substep(afid_ctx *c, int ns, double dt) {
   a.stale = c->stale;
   a.kc = c->kc;
   a.fac = c->fac_sub[ns];
+  a.bx = c->bp[0]; a.by = c->bp[1]; a.bz = c->bp[2]; a.bt = c->bp[3];
   a.sc = scalars(c);
-  {
+  if (c->split) {
+    {
+      Timed t(c, "stencil_rhs");
+      if ((rc = kerr(c, launch_stencil_rhs(a, c->st)))) return rc;
+    }
+    Timed t(c, "column_solve");
+    if ((rc = kerr(c, launch_column_solve(a, c->st)))) return rc;
+  } else {
     Timed t(c, "fused_update");
     if ((rc = kerr(c, launch_fused_update(a, c->st)))) return rc;
   }

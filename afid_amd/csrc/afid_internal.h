@@ -47,8 +47,9 @@ enum KCoef {
 // Factor rows of one wall-normal system (interleaved layout, length ldx):
 // scale = row normalisation 1/(1 - ac*betadx) (SolveImpEqnUpdate_*),
 // l = LU multipliers, rinv = 1/u, e = du/u, G/H = chunk prefix/suffix
-// products of -l / -e used by the lane scan.
-enum FacRow { FR_SCALE = 0, FR_L, FR_RINV, FR_E, FR_G, FR_H, FR_N };
+// products of -l / -e used by the lane scan, dl/du = the normalised off-
+// diagonals (unit diagonal) themselves.
+enum FacRow { FR_SCALE = 0, FR_L, FR_RINV, FR_E, FR_G, FR_H, FR_DL, FR_DU, FR_N };
 enum System { SYS_X = 0, SYS_YZ, SYS_T, SYS_N };
 
 struct StepScalars {

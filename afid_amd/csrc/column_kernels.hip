@@ -13,6 +13,8 @@
 // The order of floating-point operations inside each stencil follows the
 // reference expressions (parenthesisation kept), so per-routine results match
 // the reference to rounding.
+#include <type_traits>
+
 #include "afid_internal.h"
 #include "kernels.h"
 
@@ -208,6 +210,40 @@ __device__ __forceinline__ double chunk_solve(double *bl, const double *fac, int
   return scan_down(cnt > 0 ? fh[p] : 1.0, cnt > 0 ? acc : 0.0, p);
 }
 
+// Register-resident variant for a compile-time chunk bound LM >= L: the chunk
+// is read from LDS into b[], each sweep is unrolled so that all of its
+// coefficient loads are issued together (the recurrences then run on
+// registers), and b[] returns x without the last carry fix-up, which the
+// caller folds into its write-out: x = b[t] + H_t*cin.
+template <int LM>
+__device__ __forceinline__ double chunk_solve_reg(double (&b)[LM], const double *bl, const double *fac, int ldx,
+                                                  int cnt, int p) {
+  const double *fl = fac + FR_L * ldx, *fr = fac + FR_RINV * ldx, *fe = fac + FR_E * ldx;
+  const double *fg = fac + FR_G * ldx, *fh = fac + FR_H * ldx;
+#pragma unroll
+  for (int t = 0; t < LM; t++) b[t] = t < cnt ? bl[t * WG] : 0.0;
+  double acc = 0.0;
+#pragma unroll
+  for (int t = 0; t < LM; t++) {
+    if (t < cnt) {
+      acc = b[t] - fl[t * P + p] * acc;
+      b[t] = acc;
+    }
+  }
+  const double cin = scan_up(cnt > 0 ? fg[(cnt - 1) * P + p] : 1.0, cnt > 0 ? acc : 0.0, p);
+  acc = 0.0;
+#pragma unroll
+  for (int t = LM - 1; t >= 0; t--) {
+    if (t < cnt) {
+      const int s = t * P + p;
+      const double y = b[t] + fg[s] * cin;
+      acc = y * fr[s] - fe[s] * acc;
+      b[t] = acc;
+    }
+  }
+  return scan_down(cnt > 0 ? fh[p] : 1.0, cnt > 0 ? acc : 0.0, p);
+}
+
 // Positions of k-1 / k+1 for lane p, step t (valid only when those k exist).
 __device__ __forceinline__ int lane_skm(int p, int t, int L) { return t > 0 ? (t - 1) * P + p : (L - 1) * P + p - 1; }
 __device__ __forceinline__ int lane_skp(int p, int t, int L) { return t < L - 1 ? (t + 1) * P + p : p + 1; }
@@ -219,8 +255,464 @@ __device__ __forceinline__ int lane_skp(int p, int t, int L) { return t < L - 1 
 // rhs(1,j,i) of the reference (stale plane) is read by the vx solve and
 // replaced by the vz increment at k=1 (SolveImpEqnUpdate_YZ.F90:50), which
 // the temperature solve of the same substep then reads.
-__global__ void __launch_bounds__(WG) k_fused_update(FusedArgs a) {
+//
+// Layout of the work: a block owns a 4x4 tile of columns (wave w = z row
+// i0+w, four y columns per wave, 16 chunk lanes per column).  Tiles are
+// visited in strips 8 tiles wide in y, z fastest, and each XCD gets one
+// contiguous range of that order, so the i+-1 neighbour columns a tile reads
+// were touched a few tiles earlier on the same XCD and come from its L2.
+// Each lane walks its chunk in k keeping the k-1/k/k+1 values of the centre
+// column in registers.  The k loops carry no branches: every position a lane
+// reads lies inside the column's ldx slots (padding slots hold zeros), values
+// are computed for boundary rows too and discarded by selects, so the loads
+// of consecutive k steps overlap.  The system is solved for the new value
+// directly: A q_new = b + A q_old, with A the unit-diagonal normalised matrix,
+// which equals q_old + A^-1 b of SolveImpEqnUpdate_* and spares re-reading q_old.
+template <typename OffT>
+__device__ __forceinline__ double ldf(const double *base, OffT boff) {
+  return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(base) + boff);
+}
+template <typename OffT>
+__device__ __forceinline__ void stf(double *base, OffT boff, double v) {
+  *reinterpret_cast<double *>(reinterpret_cast<char *>(base) + boff) = v;
+}
+
+template <typename OffT>
+struct ColB {
+  OffT C, CJM, CJP, CIM, CIP, CJMIP, CJPIM;
+};
+
+constexpr int TILE_STRIP = 8;
+
+// The field pointers are separate __restrict__ parameters: the history stores
+// inside the k loops then cannot alias the stencil loads.
+template <int LM, typename OffT, bool USE_RU>
+__global__ void __launch_bounds__(WG)
+    k_fused_update(FusedArgs a, const double *__restrict__ vx, const double *__restrict__ vy,
+                   const double *__restrict__ vz, const double *__restrict__ T, const double *__restrict__ pr,
+                   double *__restrict__ rux, double *__restrict__ ruy, double *__restrict__ ruz,
+                   double *__restrict__ rut, double *__restrict__ vxn, double *__restrict__ vyn,
+                   double *__restrict__ vzn, double *__restrict__ tn, double *__restrict__ stalep,
+                   const double *__restrict__ kc, const double *__restrict__ facp) {
   extern __shared__ double lds[];
+  const Geom g = a.g;
+  const int ntj = (g.nym + 3) / 4, nti = (g.nzm + 3) / 4;
+  const int64_t lin = xcd_block((int64_t)ntj * nti);
+  if (lin < 0) return;
+  const int64_t strip = lin / ((int64_t)TILE_STRIP * nti);
+  const int64_t rr = lin - strip * TILE_STRIP * nti;
+  const int sw = min(TILE_STRIP, ntj - (int)strip * TILE_STRIP);
+  const int ti = (int)(rr / sw), tj = (int)strip * TILE_STRIP + (int)(rr % sw);
+  const int lane = threadIdx.x % 64;
+  const int p = lane % P;
+  const int j = 4 * tj + lane / P, i = 4 * ti + (int)threadIdx.x / 64;
+  const bool live = j < g.nym && i < g.nzm;
+  const int jc = live ? j : 0, ic = live ? i : 0;
+  const int jm = wrapm(jc - 1, g.nym), jp = wrapp(jc + 1, g.nym);
+  const int im = wrapm(ic - 1, g.nzm), ip = wrapp(ic + 1, g.nzm);
+  const OffT cb = (OffT)g.ldx * sizeof(double);
+  auto colb = [&](int jj, int ii) { return ((OffT)ii * (OffT)g.nym + (OffT)jj) * cb; };
+  ColB<OffT> n;
+  n.C = colb(jc, ic); n.CJM = colb(jm, ic); n.CJP = colb(jp, ic); n.CIM = colb(jc, im); n.CIP = colb(jc, ip);
+  n.CJMIP = colb(jm, ip); n.CJPIM = colb(jp, im);
+  const int64_t colc = (int64_t)ic * g.nym + jc;
+  const int L = g.L, ldx = g.ldx, nx = g.nx, nxm = g.nxm;
+  const StepScalars &c = a.sc;
+  const OffT rowb = (OffT)ldx * sizeof(double);             // one coefficient row in bytes
+  const double *kcX = kc + 0;                               // row q at kc + q*ldx
+  double *bl = lds + threadIdx.x;
+  const int k0 = p * L;
+  const int cntx = live ? max(0, min(L, nx - k0)) : 0;    // points of an nx system
+  const int cntm = live ? max(0, min(L, nxm - k0)) : 0;   // points of an nxm system
+  const OffT smb = (OffT)(k0 >= 1 ? lane_skm(p, 0, L) : p) * sizeof(double);
+  const OffT s0b = (OffT)p * sizeof(double);
+  // coefficient row byte offsets
+  const OffT rUDX3C = KC_UDX3C * rowb, rUDX3M = KC_UDX3M * rowb, rAM3CK = KC_AM3CK * rowb,
+             rAC3CK = KC_AC3CK * rowb, rAP3CK = KC_AP3CK * rowb, rAM3SK = KC_AM3SK * rowb,
+             rAC3SK = KC_AC3SK * rowb, rAP3SK = KC_AP3SK * rowb;
+  const OffT fSC = FR_SCALE * rowb, fDL = FR_DL * rowb, fDU = FR_DU * rowb;
+
+  // ---------------- vx: ImplicitAndUpdateVX + SolveImpEqnUpdate_X (n = nx)
+  {
+    const double *fac = facp + (SYS_X * FR_N) * ldx;
+    double vxc = ldf(vx, n.C + s0b);
+    double vxm = ldf(vx, n.C + smb), vym = ldf(vy, n.C + smb), vyjpm = ldf(vy, n.CJP + smb);
+    double vzm = ldf(vz, n.C + smb), vzipm = ldf(vz, n.CIP + smb), prm = ldf(pr, n.C + smb);
+    const double stale = (live && p == 0) ? stalep[colc] : 0.0;
+#pragma unroll 2
+    for (int t = 0; t < cntx; t++) {
+      const int k = k0 + t, s = t * P + p;
+      const OffT sb = (OffT)s * sizeof(double), spb = (OffT)lane_skp(p, t, L) * sizeof(double);
+      const double vxp = ldf(vx, n.C + spb);
+      const double vyc = ldf(vy, n.C + sb), vyjpc = ldf(vy, n.CJP + sb);
+      const double vzc = ldf(vz, n.C + sb), vzipc = ldf(vz, n.CIP + sb), prc = ldf(pr, n.C + sb);
+      const double vxip = ldf(vx, n.CIP + sb), vxim = ldf(vx, n.CIM + sb);
+      const double vxjp = ldf(vx, n.CJP + sb), vxjm = ldf(vx, n.CJM + sb);
+      const double tcur = ldf(T, n.C + sb);
+      const double udx3c = ldf(kcX, rUDX3C + sb);
+      const double am = ldf(kcX, rAM3CK + sb), ac = ldf(kcX, rAC3CK + sb), ap = ldf(kcX, rAP3CK + sb);
+      const double scl = ldf(fac, fSC + sb), dl = ldf(fac, fDL + sb), du = ldf(fac, fDU + sb);
+      const double ru = USE_RU ? ldf(rux, n.C + sb) : 0.0;
+      double hxz = (((vzipc + vzipm) * (vxip + vxc)) - ((vzc + vzm) * (vxc + vxim))) * c.udz4;
+      double hxy = (((vyjpc + vyjpm) * (vxjp + vxc)) - ((vyc + vym) * (vxc + vxjm))) * c.udy4;
+      double aa = vxp + vxc, bb = vxc + vxm;
+      double hxx = (aa * aa - bb * bb) * udx3c * 0.25;
+      double dzzvx = (vxim - 2.0 * vxc + vxip) * c.udzq_re;
+      double dyyvx = (vxjm - 2.0 * vxc + vxjp) * c.udyq_re;
+      double q = -(hxx + hxy + hxz) + dyyvx + dzzvx + tcur;
+      double dxxvx = vxp * ap + vxc * ac + vxm * am;
+      double dxp = (prc - prm) * (c.al * udx3c);
+      double r = USE_RU ? c.ro * ru : 0.0;
+      double rhs = (c.ga * q + r + c.alre * dxxvx - dxp) * c.dt;
+      const bool interior = k >= 1 && k <= nx - 2;
+      if (interior) stf(rux, n.C + sb, q);
+      double vin = rhs * scl + ((dl * vxm + vxc) + du * vxp);
+      double vbd = (k == 0 ? stale : 0.0) + vxc;  // identity rows: rhs(1) stale, rhs(nx) zero
+      bl[t * WG] = interior ? vin : vbd;
+      vxm = vxc; vxc = vxp; vym = vyc; vyjpm = vyjpc; vzm = vzc; vzipm = vzipc; prm = prc;
+    }
+    double b[LM];
+    const double cin = chunk_solve_reg<LM>(b, bl, fac, ldx, cntx, p);
+    const double *fh = fac + FR_H * ldx;
+#pragma unroll
+    for (int t = 0; t < LM; t++) {
+      if (t < cntx) {
+        const int k = k0 + t, s = t * P + p;
+        const double y = b[t] + fh[s] * cin;
+        stf(vxn, n.C + (OffT)s * sizeof(double), (k >= 1 && k <= nx - 2) ? y : 0.0);
+      }
+    }
+  }
+  // ---------------- vy: ImplicitAndUpdateVY + SolveImpEqnUpdate_YZ (n = nxm)
+  {
+    const double *fac = facp + (SYS_YZ * FR_N) * ldx;
+    double vxk = ldf(vx, n.C + s0b), vxjmk = ldf(vx, n.CJM + s0b), vyc = ldf(vy, n.C + s0b);
+    double vym = k0 >= 1 ? ldf(vy, n.C + smb) : vyc;
+#pragma unroll 2
+    for (int t = 0; t < cntm; t++) {
+      const int k = k0 + t, s = t * P + p;
+      const OffT sb = (OffT)s * sizeof(double), spb = (OffT)lane_skp(p, t, L) * sizeof(double);
+      const double vxkp = ldf(vx, n.C + spb), vxjmkp = ldf(vx, n.CJM + spb);
+      const double vynext = ldf(vy, n.C + spb);
+      const double vyp = k == nxm - 1 ? vyc : vynext;               // kpv clamp
+      const double vymm = k == 0 ? vyc : vym;                       // kmv clamp
+      const double vyjp = ldf(vy, n.CJP + sb), vyjm = ldf(vy, n.CJM + sb);
+      const double vyip = ldf(vy, n.CIP + sb), vyim = ldf(vy, n.CIM + sb);
+      const double vzip = ldf(vz, n.CIP + sb), vzjmip = ldf(vz, n.CJMIP + sb);
+      const double vzc = ldf(vz, n.C + sb), vzjm = ldf(vz, n.CJM + sb);
+      const double prc = ldf(pr, n.C + sb), prjm = ldf(pr, n.CJM + sb);
+      const double udx3m = ldf(kcX, rUDX3M + sb);
+      const double am = ldf(kcX, rAM3SK + sb), ac = ldf(kcX, rAC3SK + sb), ap = ldf(kcX, rAP3SK + sb);
+      const double scl = ldf(fac, fSC + sb), dl = ldf(fac, fDL + sb), du = ldf(fac, fDU + sb);
+      const double ru = USE_RU ? ldf(ruy, n.C + sb) : 0.0;
+      double hyx = ((vxkp + vxjmkp) * (vyp + vyc) - (vxk + vxjmk) * (vyc + vymm)) * udx3m * 0.25;
+      double yp = vyjp + vyc, ym = vyjm + vyc;
+      double hyy = (yp * yp - ym * ym) * c.udy4;
+      double hyz = ((vyip + vyc) * (vzip + vzjmip) - (vyc + vyim) * (vzc + vzjm)) * c.udz4;
+      double dyyvy = (vyjp - 2.0 * vyc + vyjm) * c.udyq_re;
+      double dzzvy = (vyip - 2.0 * vyc + vyim) * c.udzq_re;
+      double h = -(hyx + hyy + hyz) + dyyvy + dzzvy;
+      double dxxvy = vyp * ap + vyc * ac + vymm * am;
+      double dyp = (prc - prjm) * c.udy_al;
+      double r = USE_RU ? c.ro * ru : 0.0;
+      double rhs = (c.ga * h + r + c.alre * dxxvy - dyp) * c.dt;
+      stf(ruy, n.C + sb, h);
+      bl[t * WG] = rhs * scl + ((dl * vym + vyc) + du * vyp);
+      vym = vyc; vyc = vynext; vxk = vxkp; vxjmk = vxjmkp;
+    }
+    double b[LM];
+    const double cin = chunk_solve_reg<LM>(b, bl, fac, ldx, cntm, p);
+    const double *fh = fac + FR_H * ldx;
+#pragma unroll
+    for (int t = 0; t < LM; t++) {
+      if (t < cntx) {
+        const int k = k0 + t, s = t * P + p;
+        stf(vyn, n.C + (OffT)s * sizeof(double), k < nxm ? b[t] + fh[s] * cin : 0.0);
+      }
+    }
+  }
+  // ---------------- vz: ImplicitAndUpdateVZ + SolveImpEqnUpdate_YZ (n = nxm)
+  double dvz0 = 0.0;
+  {
+    const double *fac = facp + (SYS_YZ * FR_N) * ldx;
+    double vxk = ldf(vx, n.C + s0b), vximk = ldf(vx, n.CIM + s0b), vzc = ldf(vz, n.C + s0b);
+    double vzm = k0 >= 1 ? ldf(vz, n.C + smb) : vzc;
+    const double vz0 = vzc;
+#pragma unroll 2
+    for (int t = 0; t < cntm; t++) {
+      const int k = k0 + t, s = t * P + p;
+      const OffT sb = (OffT)s * sizeof(double), spb = (OffT)lane_skp(p, t, L) * sizeof(double);
+      const double vxkp = ldf(vx, n.C + spb), vximkp = ldf(vx, n.CIM + spb);
+      const double vznext = ldf(vz, n.C + spb);
+      const double vzp = k == nxm - 1 ? vzc : vznext;
+      const double vzmm = k == 0 ? vzc : vzm;
+      const double vzip = ldf(vz, n.CIP + sb), vzim = ldf(vz, n.CIM + sb);
+      const double vzjp = ldf(vz, n.CJP + sb), vzjm = ldf(vz, n.CJM + sb);
+      const double vyjp = ldf(vy, n.CJP + sb), vyjpim = ldf(vy, n.CJPIM + sb);
+      const double vyc = ldf(vy, n.C + sb), vyim = ldf(vy, n.CIM + sb);
+      const double prc = ldf(pr, n.C + sb), prim = ldf(pr, n.CIM + sb);
+      const double udx3m = ldf(kcX, rUDX3M + sb);
+      const double am = ldf(kcX, rAM3SK + sb), ac = ldf(kcX, rAC3SK + sb), ap = ldf(kcX, rAP3SK + sb);
+      const double scl = ldf(fac, fSC + sb), dl = ldf(fac, fDL + sb), du = ldf(fac, fDU + sb);
+      const double ru = USE_RU ? ldf(ruz, n.C + sb) : 0.0;
+      double zp = vzip + vzc, zm = vzim + vzc;
+      double hzz = (zp * zp - zm * zm) * c.udz4;
+      double hzy = ((vyjp + vyjpim) * (vzjp + vzc) - (vyc + vyim) * (vzc + vzjm)) * c.udy4;
+      double hzx = ((vxkp + vximkp) * (vzp + vzc) - (vxk + vximk) * (vzc + vzmm)) * udx3m * 0.25;
+      double dzzvz = (vzip - 2.0 * vzc + vzim) * c.udzq_re;
+      double dyyvz = (vzjp - 2.0 * vzc + vzjm) * c.udyq_re;
+      double h = -(hzx + hzy + hzz) + dyyvz + dzzvz;
+      double dxxvz = vzp * ap + vzc * ac + vzmm * am;
+      double dzp = (prc - prim) * c.dz * c.al;
+      double r = USE_RU ? c.ro * ru : 0.0;
+      double rhs = (c.ga * h + r + c.alre * dxxvz - dzp) * c.dt;
+      stf(ruz, n.C + sb, h);
+      bl[t * WG] = rhs * scl + ((dl * vzm + vzc) + du * vzp);
+      vzm = vzc; vzc = vznext; vxk = vxkp; vximk = vximkp;
+    }
+    double b[LM];
+    const double cin = chunk_solve_reg<LM>(b, bl, fac, ldx, cntm, p);
+    const double *fh = fac + FR_H * ldx;
+#pragma unroll
+    for (int t = 0; t < LM; t++) {
+      if (t < cntx) {
+        const int k = k0 + t, s = t * P + p;
+        const double y = k < nxm ? b[t] + fh[s] * cin : 0.0;
+        if (t == 0 && k == 0) dvz0 = y - vz0;
+        stf(vzn, n.C + (OffT)s * sizeof(double), y);
+      }
+    }
+  }
+  dvz0 = __shfl(dvz0, 0, P);  // lane p=0 of the column holds k=0
+  if (live && p == 0) stalep[colc] = dvz0;
+  // ---------------- temperature: SolveImpEqnUpdate_Temp (n = nx), rhs(1) = vz increment at k=1
+  {
+    const double *fac = facp + (SYS_T * FR_N) * ldx;
+    double tc = ldf(T, n.C + s0b), vxc = ldf(vx, n.C + s0b);
+    double tm = ldf(T, n.C + smb), vxm = ldf(vx, n.C + smb), vzm = ldf(vz, n.C + smb);
+    double vzipm = ldf(vz, n.CIP + smb), vym = ldf(vy, n.C + smb), vyjpm = ldf(vy, n.CJP + smb);
+#pragma unroll 2
+    for (int t = 0; t < cntx; t++) {
+      const int k = k0 + t, s = t * P + p;
+      const OffT sb = (OffT)s * sizeof(double), spb = (OffT)lane_skp(p, t, L) * sizeof(double);
+      const double tp = ldf(T, n.C + spb), vxp = ldf(vx, n.C + spb);
+      const double vzipc = ldf(vz, n.CIP + sb), vzc = ldf(vz, n.C + sb);
+      const double vyjpc = ldf(vy, n.CJP + sb), vyc = ldf(vy, n.C + sb);
+      const double tip = ldf(T, n.CIP + sb), tim = ldf(T, n.CIM + sb);
+      const double tjp = ldf(T, n.CJP + sb), tjm = ldf(T, n.CJM + sb);
+      const double udx3c = ldf(kcX, rUDX3C + sb);
+      const double am = ldf(kcX, rAM3CK + sb), ac = ldf(kcX, rAC3CK + sb), ap = ldf(kcX, rAP3CK + sb);
+      const double scl = ldf(fac, fSC + sb), dl = ldf(fac, fDL + sb), du = ldf(fac, fDU + sb);
+      const double ru = USE_RU ? ldf(rut, n.C + sb) : 0.0;
+      double htz = ((vzipm + vzipc) * (tip + tc) - (vzm + vzc) * (tc + tim)) * c.udz4;
+      double hty = ((vyjpc + vyjpm) * (tjp + tc) - (vyc + vym) * (tc + tjm)) * c.udy4;
+      double htx = ((vxp + vxc) * (tp + tc) - (vxc + vxm) * (tc + tm)) * udx3c * 0.25;
+      double dzzt = (tip - 2.0 * tc + tim) * c.udzq_pe;
+      double dyyt = (tjp - 2.0 * tc + tjm) * c.udyq_pe;
+      double h = -(htx + hty + htz) + dyyt + dzzt;
+      double dxxt = tp * ap + tc * ac + tm * am;
+      double r = USE_RU ? c.ro * ru : 0.0;
+      double rhs = (c.ga * h + r + c.alpec * dxxt) * c.dt;
+      const bool interior = k >= 1 && k <= nx - 2;
+      if (interior) stf(rut, n.C + sb, h);
+      double vin = rhs * scl + ((dl * tm + tc) + du * tp);
+      double vbd = (k == 0 ? dvz0 : 0.0) + tc;
+      bl[t * WG] = interior ? vin : vbd;
+      tm = tc; tc = tp; vxm = vxc; vxc = vxp; vzm = vzc; vzipm = vzipc; vym = vyc; vyjpm = vyjpc;
+    }
+    double b[LM];
+    const double cin = chunk_solve_reg<LM>(b, bl, fac, ldx, cntx, p);
+    const double *fh = fac + FR_H * ldx;
+#pragma unroll
+    for (int t = 0; t < LM; t++) {
+      if (t < cntx) {
+        const int k = k0 + t, s = t * P + p;
+        double y = b[t] + fh[s] * cin;
+        if (k == 0) y = c.tempbp;
+        if (k == nx - 1) y = c.temptp;
+        stf(tn, n.C + (OffT)s * sizeof(double), y);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- split update
+// The same substep update as k_fused_update in two passes.
+// k_stencil_rhs: one thread per stored point computes the explicit terms and
+// right-hand sides of all four fields (ExplicitTerms*, ImplicitAndUpdate* RHS
+// loops), updates the histories, and stores b' = ackl_b*rhs + A q_old, where A
+// is the unit-diagonal normalised system of SolveImpEqnUpdate_*; identity rows
+// store q_old (+ rhs(1) = the stale row for vx).
+// k_column_solve: a wave owns four columns (16 chunk lanes each) and solves the
+// four systems in registers: A q_new = b'.  The vz increment at k=1,
+// q_new - q_old, becomes the new stale row and the temperature's rhs(1).
+template <typename OffT, bool USE_RU>
+__global__ void __launch_bounds__(256)
+    k_stencil_rhs(FusedArgs a, const double *__restrict__ vx, const double *__restrict__ vy,
+                  const double *__restrict__ vz, const double *__restrict__ T, const double *__restrict__ pr,
+                  double *__restrict__ rux, double *__restrict__ ruy, double *__restrict__ ruz,
+                  double *__restrict__ rut, double *__restrict__ bx, double *__restrict__ by,
+                  double *__restrict__ bz, double *__restrict__ bt, const double *__restrict__ stalep,
+                  const double *__restrict__ kc, const double *__restrict__ facp) {
+  const Geom g = a.g;
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t col = gid / g.ldx;
+  const int s = (int)(gid - col * g.ldx);
+  const int L = g.L, ldx = g.ldx, nx = g.nx, nxm = g.nxm;
+  const int k = k_of_pos(s, L);
+  if (col >= g.ncol || k >= nx) return;
+  const int jc = (int)(col % g.nym), ic = (int)(col / g.nym);
+  const int jm = wrapm(jc - 1, g.nym), jp = wrapp(jc + 1, g.nym);
+  const int im = wrapm(ic - 1, g.nzm), ip = wrapp(ic + 1, g.nzm);
+  const OffT cb = (OffT)ldx * sizeof(double);
+  auto colb = [&](int jj, int ii) { return ((OffT)ii * (OffT)g.nym + (OffT)jj) * cb; };
+  const OffT C = colb(jc, ic), CJM = colb(jm, ic), CJP = colb(jp, ic), CIM = colb(jc, im), CIP = colb(jc, ip),
+             CJMIP = colb(jm, ip), CJPIM = colb(jp, im);
+  const OffT sb = (OffT)s * sizeof(double);
+  // k-1 / k+1 positions; at the column ends they point at k itself (values unused there)
+  const OffT smb = (OffT)(k >= 1 ? pos_of_k(k - 1, L) : s) * sizeof(double);
+  const OffT spb = (OffT)(k + 1 < nx ? pos_of_k(k + 1, L) : s) * sizeof(double);
+  const bool inx = k >= 1 && k <= nx - 2;   // interior rows of the nx systems
+  const bool iny = k <= nxm - 1;            // rows of the nxm systems
+  const StepScalars &c = a.sc;
+  const OffT rowb = (OffT)ldx * sizeof(double);
+  auto kcv = [&](int row) { return ldf(kc, (OffT)row * rowb + sb); };
+  // centre column, k-1 / k / k+1
+  const double vxm = ldf(vx, C + smb), vxc = ldf(vx, C + sb), vxp = ldf(vx, C + spb);
+  const double vym = ldf(vy, C + smb), vyc = ldf(vy, C + sb), vyp = ldf(vy, C + spb);
+  const double vzm = ldf(vz, C + smb), vzc = ldf(vz, C + sb), vzp = ldf(vz, C + spb);
+  const double tm = ldf(T, C + smb), tc = ldf(T, C + sb), tp = ldf(T, C + spb);
+  const double prm = ldf(pr, C + smb), prc = ldf(pr, C + sb);
+  // neighbour columns at k
+  const double vxjp = ldf(vx, CJP + sb), vxjm = ldf(vx, CJM + sb), vxip = ldf(vx, CIP + sb), vxim = ldf(vx, CIM + sb);
+  const double vyjp = ldf(vy, CJP + sb), vyjm = ldf(vy, CJM + sb), vyip = ldf(vy, CIP + sb), vyim = ldf(vy, CIM + sb);
+  const double vzjp = ldf(vz, CJP + sb), vzjm = ldf(vz, CJM + sb), vzip = ldf(vz, CIP + sb), vzim = ldf(vz, CIM + sb);
+  const double tjp = ldf(T, CJP + sb), tjm = ldf(T, CJM + sb), tip = ldf(T, CIP + sb), tim = ldf(T, CIM + sb);
+  const double prjm = ldf(pr, CJM + sb), prim = ldf(pr, CIM + sb);
+  const double vyjpim = ldf(vy, CJPIM + sb), vzjmip = ldf(vz, CJMIP + sb);
+  // neighbour columns at k-1 / k+1
+  const double vyjpm = ldf(vy, CJP + smb), vzipm = ldf(vz, CIP + smb);
+  const double vxjmp = ldf(vx, CJM + spb), vximp = ldf(vx, CIM + spb);
+  const double udx3c = kcv(KC_UDX3C), udx3m = kcv(KC_UDX3M);
+  const double amc = kcv(KC_AM3CK), acc = kcv(KC_AC3CK), apc = kcv(KC_AP3CK);
+  const double ams = kcv(KC_AM3SK), acs = kcv(KC_AC3SK), aps = kcv(KC_AP3SK);
+  const double *fX = facp + (SYS_X * FR_N) * ldx, *fY = facp + (SYS_YZ * FR_N) * ldx, *fT = facp + (SYS_T * FR_N) * ldx;
+  const OffT fSC = FR_SCALE * rowb + sb, fDL = FR_DL * rowb + sb, fDU = FR_DU * rowb + sb;
+
+  // ---- vx (ExplicitTermsVX.F90:55-101, ImplicitAndUpdateVX.F90:47-60)
+  {
+    double hxz = (((vzip + vzipm) * (vxip + vxc)) - ((vzc + vzm) * (vxc + vxim))) * c.udz4;
+    double hxy = (((vyjp + vyjpm) * (vxjp + vxc)) - ((vyc + vym) * (vxc + vxjm))) * c.udy4;
+    double aa = vxp + vxc, bb = vxc + vxm;
+    double hxx = (aa * aa - bb * bb) * udx3c * 0.25;
+    double dzzvx = (vxim - 2.0 * vxc + vxip) * c.udzq_re;
+    double dyyvx = (vxjm - 2.0 * vxc + vxjp) * c.udyq_re;
+    double q = -(hxx + hxy + hxz) + dyyvx + dzzvx + tc;
+    double dxxvx = vxp * apc + vxc * acc + vxm * amc;
+    double dxp = (prc - prm) * (c.al * udx3c);
+    double r = USE_RU ? c.ro * ldf(rux, C + sb) : 0.0;
+    double rhs = (c.ga * q + r + c.alre * dxxvx - dxp) * c.dt;
+    double vin = rhs * ldf(fX, fSC) + ((ldf(fX, fDL) * vxm + vxc) + ldf(fX, fDU) * vxp);
+    double vbd = (k == 0 ? stalep[col] : 0.0) + vxc;
+    if (inx) stf(rux, C + sb, q);
+    stf(bx, C + sb, inx ? vin : vbd);
+  }
+  // ---- vy (ExplicitTermsVY.F90:52-98, ImplicitAndUpdateVY.F90:52-68); kmv/kpv clamps
+  {
+    const double vymm = k == 0 ? vyc : vym, vypp = k == nxm - 1 ? vyc : vyp;
+    double hyx = ((vxp + vxjmp) * (vypp + vyc) - (vxc + vxjm) * (vyc + vymm)) * udx3m * 0.25;
+    double yp = vyjp + vyc, ym = vyjm + vyc;
+    double hyy = (yp * yp - ym * ym) * c.udy4;
+    double hyz = ((vyip + vyc) * (vzip + vzjmip) - (vyc + vyim) * (vzc + vzjm)) * c.udz4;
+    double dyyvy = (vyjp - 2.0 * vyc + vyjm) * c.udyq_re;
+    double dzzvy = (vyip - 2.0 * vyc + vyim) * c.udzq_re;
+    double h = -(hyx + hyy + hyz) + dyyvy + dzzvy;
+    double dxxvy = vypp * aps + vyc * acs + vymm * ams;
+    double dyp = (prc - prjm) * c.udy_al;
+    double r = USE_RU ? c.ro * ldf(ruy, C + sb) : 0.0;
+    double rhs = (c.ga * h + r + c.alre * dxxvy - dyp) * c.dt;
+    double dl = ldf(fY, fDL), du = ldf(fY, fDU);
+    double vin = rhs * ldf(fY, fSC) + ((dl * vym + vyc) + du * vyp);
+    if (iny) {
+      stf(ruy, C + sb, h);
+      stf(by, C + sb, vin);
+    }
+  }
+  // ---- vz (ExplicitTermsVZ.F90:56-101, ImplicitAndUpdateVZ.F90:48-64)
+  {
+    const double vzmm = k == 0 ? vzc : vzm, vzpp = k == nxm - 1 ? vzc : vzp;
+    double zp = vzip + vzc, zm = vzim + vzc;
+    double hzz = (zp * zp - zm * zm) * c.udz4;
+    double hzy = ((vyjp + vyjpim) * (vzjp + vzc) - (vyc + vyim) * (vzc + vzjm)) * c.udy4;
+    double hzx = ((vxp + vximp) * (vzpp + vzc) - (vxc + vxim) * (vzc + vzmm)) * udx3m * 0.25;
+    double dzzvz = (vzip - 2.0 * vzc + vzim) * c.udzq_re;
+    double dyyvz = (vzjp - 2.0 * vzc + vzjm) * c.udyq_re;
+    double h = -(hzx + hzy + hzz) + dyyvz + dzzvz;
+    double dxxvz = vzpp * aps + vzc * acs + vzmm * ams;
+    double dzp = (prc - prim) * c.dz * c.al;
+    double r = USE_RU ? c.ro * ldf(ruz, C + sb) : 0.0;
+    double rhs = (c.ga * h + r + c.alre * dxxvz - dzp) * c.dt;
+    double dl = ldf(fY, fDL), du = ldf(fY, fDU);
+    double vin = rhs * ldf(fY, fSC) + ((dl * vzm + vzc) + du * vzp);
+    if (iny) {
+      stf(ruz, C + sb, h);
+      stf(bz, C + sb, vin);
+    }
+  }
+  // ---- temperature (ExplicitTermsTemp.F90:52-93, ImplicitAndUpdateTemp.F90:41-54)
+  {
+    double htz = ((vzipm + vzip) * (tip + tc) - (vzm + vzc) * (tc + tim)) * c.udz4;
+    double hty = ((vyjp + vyjpm) * (tjp + tc) - (vyc + vym) * (tc + tjm)) * c.udy4;
+    double htx = ((vxp + vxc) * (tp + tc) - (vxc + vxm) * (tc + tm)) * udx3c * 0.25;
+    double dzzt = (tip - 2.0 * tc + tim) * c.udzq_pe;
+    double dyyt = (tjp - 2.0 * tc + tjm) * c.udyq_pe;
+    double h = -(htx + hty + htz) + dyyt + dzzt;
+    double dxxt = tp * apc + tc * acc + tm * amc;
+    double r = USE_RU ? c.ro * ldf(rut, C + sb) : 0.0;
+    double rhs = (c.ga * h + r + c.alpec * dxxt) * c.dt;
+    double vin = rhs * ldf(fT, fSC) + ((ldf(fT, fDL) * tm + tc) + ldf(fT, fDU) * tp);
+    if (inx) stf(rut, C + sb, h);
+    stf(bt, C + sb, inx ? vin : tc);   // row 1 gets the vz increment added by the solve pass
+  }
+}
+
+// Loads lane p's chunk of b' into registers, solves, returns cin (x = b + H cin).
+template <int LM, typename OffT>
+__device__ __forceinline__ double solve_chunk_from(double (&b)[LM], const double *__restrict__ src, OffT C,
+                                                   const double *fac, int ldx, int cnt, int p) {
+  const double *fl = fac + FR_L * ldx, *fr = fac + FR_RINV * ldx, *fe = fac + FR_E * ldx;
+  const double *fg = fac + FR_G * ldx, *fh = fac + FR_H * ldx;
+#pragma unroll
+  for (int t = 0; t < LM; t++) b[t] = t < cnt ? ldf(src, C + (OffT)(t * P + p) * sizeof(double)) : 0.0;
+  double acc = 0.0;
+#pragma unroll
+  for (int t = 0; t < LM; t++) {
+    if (t < cnt) {
+      acc = b[t] - fl[t * P + p] * acc;
+      b[t] = acc;
+    }
+  }
+  const double cin = scan_up(cnt > 0 ? fg[(cnt - 1) * P + p] : 1.0, cnt > 0 ? acc : 0.0, p);
+  acc = 0.0;
+#pragma unroll
+  for (int t = LM - 1; t >= 0; t--) {
+    if (t < cnt) {
+      const int s = t * P + p;
+      const double y = b[t] + fg[s] * cin;
+      acc = y * fr[s] - fe[s] * acc;
+      b[t] = acc;
+    }
+  }
+  return scan_down(cnt > 0 ? fh[p] : 1.0, cnt > 0 ? acc : 0.0, p);
+}
+
+template <int LM, typename OffT>
+__global__ void __launch_bounds__(256)
+    k_column_solve(FusedArgs a, const double *__restrict__ vz, const double *__restrict__ T,
+                   double *__restrict__ bx, double *__restrict__ by, double *__restrict__ bz,
+                   double *__restrict__ bt, double *__restrict__ vxn, double *__restrict__ vyn,
+                   double *__restrict__ vzn, double *__restrict__ tn, double *__restrict__ stalep,
+                   const double *__restrict__ facp) {
   const Geom g = a.g;
   const int64_t blk = xcd_block((g.ncol + COLS_PER_WG - 1) / COLS_PER_WG);
   if (blk < 0) return;
@@ -228,108 +720,65 @@ __global__ void __launch_bounds__(WG) k_fused_update(FusedArgs a) {
   const int64_t col = blk * COLS_PER_WG + threadIdx.x / P;
   const bool live = col < g.ncol;
   const int64_t colc = live ? col : 0;
-  const int j = (int)(colc % g.nym), i = (int)(colc / g.nym);
-  const Nb n = make_nb(j, i, g);
+  const OffT C = (OffT)colc * (OffT)g.ldx * sizeof(double);
   const int L = g.L, ldx = g.ldx, nx = g.nx, nxm = g.nxm;
-  const StepScalars &c = a.sc;
-  const double *kc = a.kc;
-  double *bl = lds + threadIdx.x;
   const int k0 = p * L;
-  const int cntx = live ? max(0, min(L, nx - k0)) : 0;    // points of an nx system
-  const int cntm = live ? max(0, min(L, nxm - k0)) : 0;   // points of an nxm system
-
-  // ---- vx: ImplicitAndUpdateVX + SolveImpEqnUpdate_X (n = nx, rows 1 and nx identity)
-  {
-    const double *fac = a.fac + (SYS_X * FR_N) * ldx;
-    for (int t = 0; t < cntx; t++) {
-      const int k = k0 + t, s = t * P + p;
-      double v = 0.0;
-      if (k >= 1 && k <= nx - 2) {
-        const int skm = lane_skm(p, t, L), skp = lane_skp(p, t, L);
-        double q = expl_vx(a.f, n, s, skm, skp, kc[KC_UDX3C * ldx + s], c);
-        double ru = c.use_ru ? a.rux[n.C + s] : 0.0;
-        v = rhs_vx(q, ru, a.f.vx, a.f.pr, n, s, skm, skp, kc, ldx, c) * fac[FR_SCALE * ldx + s];
-        a.rux[n.C + s] = q;
-      } else if (k == 0) {
-        v = a.stale[colc];
+  const int cntx = live ? max(0, min(L, nx - k0)) : 0;
+  const int cntm = live ? max(0, min(L, nxm - k0)) : 0;
+  double b[LM];
+  {  // vx
+    const double *fac = facp + (SYS_X * FR_N) * ldx, *fh = fac + FR_H * ldx;
+    const double cin = solve_chunk_from<LM, OffT>(b, bx, C, fac, ldx, cntx, p);
+#pragma unroll
+    for (int t = 0; t < LM; t++)
+      if (t < cntx) {
+        const int k = k0 + t, s = t * P + p;
+        stf(vxn, C + (OffT)s * sizeof(double), (k >= 1 && k <= nx - 2) ? b[t] + fh[s] * cin : 0.0);
       }
-      bl[t * WG] = v;
-    }
-    const double cin = chunk_solve(bl, fac, ldx, cntx, p);
-    const double *fh = fac + FR_H * ldx;
-    for (int t = 0; t < cntx; t++) {
-      const int k = k0 + t, s = t * P + p;
-      const double x = bl[t * WG] + fh[s] * cin;
-      a.vxn[n.C + s] = (k >= 1 && k <= nx - 2) ? a.f.vx[n.C + s] + x : 0.0;
-    }
   }
-  // ---- vy, vz: SolveImpEqnUpdate_YZ (n = nxm)
+  {  // vy
+    const double *fac = facp + (SYS_YZ * FR_N) * ldx, *fh = fac + FR_H * ldx;
+    const double cin = solve_chunk_from<LM, OffT>(b, by, C, fac, ldx, cntm, p);
+#pragma unroll
+    for (int t = 0; t < LM; t++)
+      if (t < cntx) {
+        const int k = k0 + t, s = t * P + p;
+        stf(vyn, C + (OffT)s * sizeof(double), k < nxm ? b[t] + fh[s] * cin : 0.0);
+      }
+  }
   double dvz0 = 0.0;
-  for (int comp = 0; comp < 2; comp++) {
-    const double *fac = a.fac + (SYS_YZ * FR_N) * ldx;
-    const double *q0 = comp == 0 ? a.f.vy : a.f.vz;
-    double *ru = comp == 0 ? a.ruy : a.ruz;
-    double *qn = comp == 0 ? a.vyn : a.vzn;
-    for (int t = 0; t < cntm; t++) {
-      const int k = k0 + t, s = t * P + p;
-      const int skp = lane_skp(p, t, L);
-      const int skmm = k == 0 ? s : lane_skm(p, t, L);
-      const int skpp = k == nxm - 1 ? s : skp;
-      const double udx3m = kc[KC_UDX3M * ldx + s];
-      const double r0 = c.use_ru ? ru[n.C + s] : 0.0;
-      double h, v;
-      if (comp == 0) {
-        h = expl_vy(a.f, n, s, skp, skmm, skpp, udx3m, c);
-        v = rhs_vy(h, r0, q0, a.f.pr, n, s, skmm, skpp, kc, ldx, c);
-      } else {
-        h = expl_vz(a.f, n, s, skp, skmm, skpp, udx3m, c);
-        v = rhs_vz(h, r0, q0, a.f.pr, n, s, skmm, skpp, kc, ldx, c);
+  {  // vz
+    const double *fac = facp + (SYS_YZ * FR_N) * ldx, *fh = fac + FR_H * ldx;
+    const double cin = solve_chunk_from<LM, OffT>(b, bz, C, fac, ldx, cntm, p);
+    const double vz0 = (live && p == 0) ? ldf(vz, C) : 0.0;   // position of k = 0 is slot 0
+#pragma unroll
+    for (int t = 0; t < LM; t++)
+      if (t < cntx) {
+        const int k = k0 + t, s = t * P + p;
+        const double y = k < nxm ? b[t] + fh[s] * cin : 0.0;
+        if (t == 0 && k == 0) dvz0 = y - vz0;
+        stf(vzn, C + (OffT)s * sizeof(double), y);
       }
-      bl[t * WG] = v * fac[FR_SCALE * ldx + s];
-      ru[n.C + s] = h;
-    }
-    const double cin = chunk_solve(bl, fac, ldx, cntm, p);
-    const double *fh = fac + FR_H * ldx;
-    for (int t = 0; t < cntx; t++) {
-      const int k = k0 + t, s = t * P + p;
-      if (k < nxm) {
-        const double x = bl[t * WG] + fh[s] * cin;
-        qn[n.C + s] = q0[n.C + s] + x;
-        if (comp == 1 && k == 0) dvz0 = x;
-      } else {
-        qn[n.C + s] = 0.0;
-      }
-    }
   }
-  dvz0 = __shfl(dvz0, 0, P);  // lane p=0 of the column holds k=0
-  if (live && p == 0) a.stale[colc] = dvz0;
-  // ---- temperature: SolveImpEqnUpdate_Temp (n = nx), rhs(1) = the vz increment at k=1
-  {
-    const double *fac = a.fac + (SYS_T * FR_N) * ldx;
-    for (int t = 0; t < cntx; t++) {
-      const int k = k0 + t, s = t * P + p;
-      double v = 0.0;
-      if (k >= 1 && k <= nx - 2) {
-        const int skm = lane_skm(p, t, L), skp = lane_skp(p, t, L);
-        double h = expl_temp(a.f, n, s, skm, skp, kc[KC_UDX3C * ldx + s], c);
-        double ru = c.use_ru ? a.rut[n.C + s] : 0.0;
-        v = rhs_temp(h, ru, a.f.temp, n, s, skm, skp, kc, ldx, c) * fac[FR_SCALE * ldx + s];
-        a.rut[n.C + s] = h;
-      } else if (k == 0) {
-        v = dvz0;
-      }
-      bl[t * WG] = v;
-    }
-    const double cin = chunk_solve(bl, fac, ldx, cntx, p);
-    const double *fh = fac + FR_H * ldx;
-    for (int t = 0; t < cntx; t++) {
-      const int k = k0 + t, s = t * P + p;
-      double v = a.f.temp[n.C + s] + (bl[t * WG] + fh[s] * cin);
-      if (k == 0) v = c.tempbp;
-      if (k == nx - 1) v = c.temptp;
-      a.tn[n.C + s] = v;
-    }
+  dvz0 = __shfl(dvz0, 0, P);
+  if (live && p == 0) {
+    stalep[colc] = dvz0;
+    stf(bt, C, ldf(bt, C) + dvz0);   // temperature row 1: rhs(1) = vz increment
   }
+  {  // temperature
+    const double *fac = facp + (SYS_T * FR_N) * ldx, *fh = fac + FR_H * ldx;
+    const double cin = solve_chunk_from<LM, OffT>(b, bt, C, fac, ldx, cntx, p);
+#pragma unroll
+    for (int t = 0; t < LM; t++)
+      if (t < cntx) {
+        const int k = k0 + t, s = t * P + p;
+        double y = b[t] + fh[s] * cin;
+        if (k == 0) y = a.sc.tempbp;
+        if (k == nx - 1) y = a.sc.temptp;
+        stf(tn, C + (OffT)s * sizeof(double), y);
+      }
+  }
+  (void)T;
 }
 
 // ---------------------------------------------------------------- per-routine kernels
@@ -505,10 +954,29 @@ __global__ void k_correct(CorrectArgs a, int which) {
 // ---------------------------------------------------------------- launchers
 static size_t column_lds(const Geom &g) { return sizeof(double) * WG * (size_t)g.L; }
 
+#define AFID_LM_LIST(X) X(2) X(4) X(8) X(12) X(16) X(25) X(32) X(48) X(64) X(80)
+
+static int pick_lm(int L) {
+  const int lms[] = {2, 4, 8, 12, 16, 25, 32, 48, 64, 80};
+  for (int v : lms)
+    if (L <= v) return v;
+  return -1;
+}
+
+#define AFID_FUSED_VARIANTS(X, V) \
+  X(V, uint32_t, true) X(V, uint32_t, false) X(V, uint64_t, true) X(V, uint64_t, false)
+
 hipError_t column_configure_lds(size_t bytes) {
-  hipError_t e = hipFuncSetAttribute((const void *)k_fused_update, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  hipError_t e = hipSuccess;
+#define SETA1(V, O, R)                                                                                   \
+  e = hipFuncSetAttribute((const void *)k_fused_update<V, O, R>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                          (int)bytes);                                                                   \
   if (e != hipSuccess) return e;
-  return hipFuncSetAttribute((const void *)k_solve_update, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+#define SETA(V) AFID_FUSED_VARIANTS(SETA1, V)
+  AFID_LM_LIST(SETA)
+#undef SETA
+#undef SETA1
+  return hipSuccess;
 }
 
 static dim3 column_grid(const Geom &g) {
@@ -517,8 +985,66 @@ static dim3 column_grid(const Geom &g) {
 }
 
 hipError_t launch_fused_update(const FusedArgs &a, hipStream_t st) {
-  hipLaunchKernelGGL(k_fused_update, column_grid(a.g), dim3(WG), column_lds(a.g), st, a);
-  return hipGetLastError();
+  const int64_t ntiles = (int64_t)((a.g.nym + 3) / 4) * ((a.g.nzm + 3) / 4);
+  const dim3 grid((unsigned)(((ntiles + 7) / 8) * 8));
+  // 32-bit byte offsets against the uniform base pointers while a field fits in 4 GiB
+  const bool small = (uint64_t)a.g.ncol * a.g.ldx * sizeof(double) < (1ull << 32);
+  const bool ru = a.sc.use_ru != 0;
+#define LAUNCH1(V, O, R)                                                                                    \
+  if (std::is_same<O, uint32_t>::value == small && R == ru) {                                              \
+    hipLaunchKernelGGL((k_fused_update<V, O, R>), grid, dim3(WG), column_lds(a.g), st, a, a.f.vx, a.f.vy, \
+                       a.f.vz, a.f.temp, a.f.pr, a.rux, a.ruy, a.ruz, a.rut, a.vxn, a.vyn, a.vzn, a.tn,  \
+                       a.stale, a.kc, a.fac);                                                              \
+    return hipGetLastError();                                                                              \
+  }
+  switch (pick_lm(a.g.L)) {
+#define CASE(V)                        \
+  case V: {                            \
+    AFID_FUSED_VARIANTS(LAUNCH1, V)    \
+    break;                             \
+  }
+    AFID_LM_LIST(CASE)
+#undef CASE
+#undef LAUNCH1
+    default: break;
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_stencil_rhs(const FusedArgs &a, hipStream_t st) {
+  const int64_t n = a.g.ncol * a.g.ldx;
+  const dim3 grid((unsigned)((n + 255) / 256));
+  const bool small = (uint64_t)n * sizeof(double) < (1ull << 32);
+  const bool ru = a.sc.use_ru != 0;
+#define SR(O, R)                                                                                         \
+  if (std::is_same<O, uint32_t>::value == small && R == ru) {                                          \
+    hipLaunchKernelGGL((k_stencil_rhs<O, R>), grid, dim3(256), 0, st, a, a.f.vx, a.f.vy, a.f.vz, a.f.temp, \
+                       a.f.pr, a.rux, a.ruy, a.ruz, a.rut, a.bx, a.by, a.bz, a.bt, a.stale, a.kc, a.fac);   \
+    return hipGetLastError();                                                                          \
+  }
+  SR(uint32_t, true) SR(uint32_t, false) SR(uint64_t, true) SR(uint64_t, false)
+#undef SR
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_column_solve(const FusedArgs &a, hipStream_t st) {
+  const bool small = (uint64_t)a.g.ncol * a.g.ldx * sizeof(double) < (1ull << 32);
+  const dim3 grid = column_grid(a.g);
+  switch (pick_lm(a.g.L)) {
+#define CASE(V)                                                                                        \
+  case V:                                                                                              \
+    if (small)                                                                                         \
+      hipLaunchKernelGGL((k_column_solve<V, uint32_t>), grid, dim3(256), 0, st, a, a.f.vz, a.f.temp, a.bx, \
+                         a.by, a.bz, a.bt, a.vxn, a.vyn, a.vzn, a.tn, a.stale, a.fac);                  \
+    else                                                                                               \
+      hipLaunchKernelGGL((k_column_solve<V, uint64_t>), grid, dim3(256), 0, st, a, a.f.vz, a.f.temp, a.bx, \
+                         a.by, a.bz, a.bt, a.vxn, a.vyn, a.vzn, a.tn, a.stale, a.fac);                  \
+    return hipGetLastError();
+    AFID_LM_LIST(CASE)
+#undef CASE
+    default: break;
+  }
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_solve_update(const SolveArgs &a, int which, hipStream_t st) {

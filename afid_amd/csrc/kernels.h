@@ -12,6 +12,7 @@ struct FusedArgs {
   double *stale;                     // rhs(1,j,i) of the reference, nym*nzm
   const double *kc;                  // KC_N interleaved coefficient rows
   const double *fac;                 // SYS_N*FR_N interleaved factor rows
+  double *bx, *by, *bz, *bt;         // right-hand sides b + A q_old of the split update
   StepScalars sc;
 };
 
@@ -78,6 +79,8 @@ struct PoissonArgs {
 };
 
 hipError_t launch_fused_update(const FusedArgs &a, hipStream_t st);
+hipError_t launch_stencil_rhs(const FusedArgs &a, hipStream_t st);
+hipError_t launch_column_solve(const FusedArgs &a, hipStream_t st);
 hipError_t launch_solve_update(const SolveArgs &a, int which, hipStream_t st);
 hipError_t launch_explicit(const RoutineArgs &a, int which, hipStream_t st);
 hipError_t launch_rhs(const RoutineArgs &a, int which, hipStream_t st);

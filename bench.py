@@ -22,17 +22,24 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# Algorithmic FP64 words per cell (compulsory traffic of the reference routines):
-# explicit VX 5, VY 4, VZ 4, Temp 5; implicit VX/VY/VZ 6 each, Temp 5;
+# Algorithmic FP64 words per cell of each kernel: the distinct fields it must
+# read and write once (8 B each).  stencil_rhs reads vx,vy,vz,T,pr and the four
+# histories and writes the histories and the four right-hand sides b + A q_old;
+# column_solve reads those and writes the four new fields; fused_update is the
+# two in one pass.  Spectral arrays hold nym/2+1 complex values per y line, one
+# word per cell.  The substep figure is the reference's per-routine compulsory
+# traffic: explicit VX 5, VY 4, VZ 4, Temp 5; implicit VX/VY/VZ 6 each, Temp 5;
 # divergence 4; pressure solve 10 (5 passes); CorrectVelocity 7; CorrectPressure 3.
 WORDS = {
-    "fused_update": 5 + 4 + 4 + 5 + 6 + 6 + 6 + 5,
-    "div_yfft_r2c": 4 + 2,
+    "stencil_rhs": 9 + 8,
+    "column_solve": 4 + 4,
+    "fused_update": 9 + 4 + 4,
+    "div_yfft_r2c": 3 + 1,
     "zfft_fwd": 2,
     "xsolve_pressure": 2,
     "zfft_bwd": 2,
     "yfft_c2r": 2,
-    "correct_vel_pr": 7 + 3,
+    "correct_vel_pr": 5 + 4,
 }
 SUBSTEP_WORDS = 65
 

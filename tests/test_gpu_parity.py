@@ -231,3 +231,25 @@ def test_conduction_fixed_point_on_gpu(afid):
     assert abs(nb - 1) < 1e-12 and abs(nt - 1) < 1e-12
     assert abs(cell.GlobalQuantities()[0] - 1) < 1e-12
     cell.close()
+
+
+@pytest.mark.parametrize("c", SMALL_CASES, ids=case_id)
+def test_fused_and_split_update_agree(afid, c, monkeypatch):
+    """The one-pass update (AFID_UPDATE=fused) and the default stencil + column
+    solve pair compute the same substep; both are checked against the oracle."""
+    o, _ = make_pair(afid, c)
+    o.init_linear_noise(5, 1e-3)
+    out = {}
+    for path in ("split", "fused"):
+        monkeypatch.setenv("AFID_UPDATE", path)
+        cell = afid.Cell(afid.CellParams(nsst=3, alx3=1.0, **c))
+        cell.init_linear_noise(5, 1e-3)
+        for n in range(4):
+            cell.step(n, 1, 2e-3, 1e-4, 5e-3, 1.2)
+        out[path] = {nm: cell.download(nm) for nm in ("vx", "vy", "vz", "temp", "pr")}
+        cell.close()
+    for n in range(4):
+        o.step(n, 1, 2e-3, 1e-4, 5e-3, 1.2)
+    for nm in ("vx", "vy", "vz", "temp"):
+        assert rel_l2(out["split"][nm], out["fused"][nm]) < 1e-13, nm
+        assert rel_l2(out["split"][nm], o.interior(nm)) < 1e-11, nm
