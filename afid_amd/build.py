@@ -1,7 +1,10 @@
-"""Builds libafid_hip.so (the HIP engine behind include/afid.h) in-tree.
+"""Builds libafid_hip.so (the HIP engine behind include/afid.h) in-tree, and
+the Fortran host afid_run on top of it.
 
 hipcc compiles each translation unit for gfx950 and links one shared library
 into afid_amd/lib/.  Packed-FP32 instructions are disabled for all device code.
+amdflang compiles afid_amd/fortran/ (iso_c_binding module + main program)
+into afid_amd/lib/afid_run, linked against the library with an $ORIGIN rpath.
 """
 from __future__ import annotations
 
@@ -51,7 +54,27 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")
     os.replace(tmp, LIB)
+    build_fortran_host()
     return LIB
+
+
+FORTRAN_SOURCES = ["afid_hip.F90", "afid_main.F90"]
+RUN = os.path.join(LIBDIR, "afid_run")
+
+
+def build_fortran_host() -> str:
+    fc = os.environ.get("AFID_FC", "/opt/rocm/bin/amdflang")
+    fdir = os.path.join(HERE, "fortran")
+    moddir = os.path.join(LIBDIR, "obj", "fmod")
+    os.makedirs(moddir, exist_ok=True)
+    tmp = RUN + ".tmp"
+    cmd = [fc, "-cpp", "-O2", "-J", moddir, *[os.path.join(fdir, f) for f in FORTRAN_SOURCES],
+           f"-L{LIBDIR}", "-lafid_hip", "-Wl,-rpath,$ORIGIN", "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"amdflang failed:\n{r.stderr[-4000:]}")
+    os.replace(tmp, RUN)
+    return RUN
 
 
 if __name__ == "__main__":

@@ -90,6 +90,8 @@ def make_stages(name, p, ns, dt, wd, seed=5):
     out["nu_vol"] = float(open(os.path.join(wd, "nu_vol.out")).read().split()[1])
     nud = open(os.path.join(wd, "nu_diss.out")).read().split()
     out["nu_diss"] = np.array([float(nud[1]), float(nud[2])])
+    nup = open(os.path.join(wd, "nu_plate.out")).read().split()   # format 4(1x,e14.6): 6 digits
+    out["nu_plate"] = np.array([float(nup[1]), float(nup[2])])
     for fn in ("nu_vol.out", "nu_diss.out", "nu_plate.out", "rms_vel.out"):
         os.remove(os.path.join(wd, fn))
     np.savez_compressed(os.path.join(HERE, f"stages_{name}.npz"), ns=ns, dt=dt, dph_in=dphin,

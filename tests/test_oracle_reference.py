@@ -6,7 +6,7 @@ What the fixtures pin: CreateGrid; ExplicitTerms{VX,VY,VZ,Temp};
 ImplicitAndUpdate{VX,VY,VZ,Temp} with SolveImpEqnUpdate_{X,YZ,Temp} (dgttrf/
 dgttrs from MKL) including the stale rhs(1) row; CalcLocalDivergence;
 CorrectVelocity; CorrectPressure; CalcMaxCFL; CheckDivergence;
-GlobalQuantities; CalcDissipationNu; and 10 composed TimeMarcher steps with
+GlobalQuantities; CalcDissipationNu; CalcPlateNu (6 digits); and 10 composed TimeMarcher steps with
 main.F90's dt selection.  The reference pressure solve needs FFTW3, which is
 absent, so the composed march runs the oracle's Poisson solve inside the
 reference TimeMarcher; the Poisson solve itself is pinned in
@@ -105,6 +105,8 @@ def test_stages_match_reference(case):
     np.testing.assert_allclose(gq[4:7], E[2:5], rtol=1e-15)            # vmax(1..3)
     np.testing.assert_allclose(gq[1:4], E[5:8], rtol=1e-13)            # tempm, tempmax, tempmin
     assert gq[0] == pytest.approx(float(d["nu_vol"]), rel=1e-13)
+    # nu_plate.out holds 6 significant digits (format e14.6, CalcPlateNu.F90:51-52)
+    np.testing.assert_allclose(o.plate_nu(), d["nu_plate"], rtol=5e-6)
     mu, th, de, dth = o.dissipation()
     np.testing.assert_allclose([mu, th], d["nu_diss"], rtol=1e-13)
     np.testing.assert_allclose(de, E[8:8 + nxm], rtol=1e-13)
