@@ -239,6 +239,8 @@ bool fft_factors(int n, int *f, int *nf) {
 int make_plan(afid_ctx *c, int n, FFTPlan *pl, DevPlan *dp, int realN) {
   pl->n = n;
   if (!fft_factors(n, pl->f, &pl->nf)) return AFID_ERR_FFTSIZE;
+  for (int s = 0, m = 1; s < pl->nf; m *= pl->f[s], s++)
+    pl->mdiv[s] = m == 1 ? 0u : (unsigned)(((1ull << 32) + (uint64_t)m - 1) / (uint64_t)m);
   std::vector<double2> tw(n);
   const long double twopi = 6.283185307179586476925286766559005768L;
   for (int t = 0; t < n; t++) {

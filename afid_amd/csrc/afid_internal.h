@@ -37,6 +37,30 @@ __host__ __device__ inline int64_t col_off(int j, int i, const Geom &g) {
 __host__ __device__ inline int wrapm(int a, int n) { return a < 0 ? a + n : a; }
 __host__ __device__ inline int wrapp(int a, int n) { return a >= n ? a - n : a; }
 
+// XCD-aware block order: the hardware deals blocks b, b+8, b+16 ... to one
+// XCD, so each XCD is given one contiguous range of the logical block order
+// (grid padded to a multiple of 8).  Returns -1 for padding blocks.
+__device__ __forceinline__ int64_t xcd_block(int64_t nblk) {
+  int64_t b = blockIdx.x;
+  int64_t per = (gridDim.x + 7) / 8;
+  int64_t nb = (b % 8) * per + b / 8;
+  return nb < nblk ? nb : -1;
+}
+
+// Logical column order of the point kernels: strips STRIP_W columns wide in y,
+// z next, y within the strip fastest.  A column's y and z neighbours are then
+// at most STRIP_W columns away in that order and are read from the L2 of the
+// XCD that streams the strip.
+constexpr int STRIP_W = 16;
+__device__ __forceinline__ void strip_col(int64_t c, const Geom &g, int &j, int &i) {
+  const int64_t per_strip = (int64_t)STRIP_W * g.nzm;
+  const int strip = (int)(c / per_strip);
+  const int r = (int)(c - strip * per_strip);
+  const int sw = min(STRIP_W, g.nym - strip * STRIP_W);
+  i = r / sw;
+  j = strip * STRIP_W + (r - i * sw);
+}
+
 // Per-k coefficient rows held on device in the interleaved layout: row q of
 // the block kc is kc[q*ldx + pos(k)].
 enum KCoef {

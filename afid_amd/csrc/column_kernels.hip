@@ -38,15 +38,6 @@ __device__ __forceinline__ Nb make_nb(int j, int i, const Geom &g) {
   return n;
 }
 
-// XCD-aware block order: blocks b, b+8, b+16 ... land on one XCD, so give
-// each XCD one contiguous range of columns (neighbouring columns share L2).
-__device__ __forceinline__ int64_t xcd_block(int64_t nblk) {
-  int64_t b = blockIdx.x;
-  int64_t per = (gridDim.x + 7) / 8;
-  int64_t nb = (b % 8) * per + b / 8;
-  return nb < nblk ? nb : -1;
-}
-
 // ---------------------------------------------------------------- explicit terms
 // ExplicitTermsVX.F90:55-101 at interior point k (1..nx-2)
 __device__ __forceinline__ double expl_vx(const FieldPtrs &f, const Nb &n, int s, int skm, int skp,
@@ -538,14 +529,20 @@ __global__ void __launch_bounds__(WG)
 
 // ---------------------------------------------------------------- split update
 // The same substep update as k_fused_update in two passes.
-// k_stencil_rhs: one thread per stored point computes the explicit terms and
-// right-hand sides of all four fields (ExplicitTerms*, ImplicitAndUpdate* RHS
-// loops), updates the histories, and stores b' = ackl_b*rhs + A q_old, where A
-// is the unit-diagonal normalised system of SolveImpEqnUpdate_*; identity rows
-// store q_old (+ rhs(1) = the stale row for vx).
+// k_stencil_rhs: one block per column (strip order, XCD-remapped, so the y/z
+// neighbour columns it reads were just read by nearby blocks of the same XCD);
+// each thread takes stored points of the column and computes the explicit
+// terms and right-hand sides of all four fields (ExplicitTerms*,
+// ImplicitAndUpdate* RHS loops), updates the histories, and stores
+// b' = ackl_b*rhs + A q_old, where A is the unit-diagonal normalised system of
+// SolveImpEqnUpdate_*; identity rows store q_old (+ rhs(1) = the stale row for vx).
+// Every store is unconditional (selects instead of branches) so that all loads
+// of a point issue in one batch; the history rows outside the reference loops
+// (k = 1 and nx for vx/temp, nx for vy/vz), which nothing reads, are kept at 0.
 // k_column_solve: a wave owns four columns (16 chunk lanes each) and solves the
-// four systems in registers: A q_new = b'.  The vz increment at k=1,
-// q_new - q_old, becomes the new stale row and the temperature's rhs(1).
+// four systems in registers: A q_new = b', with the factor rows staged in LDS.
+// The vz increment at k=1, q_new - q_old, becomes the new stale row and the
+// temperature's rhs(1).
 template <typename OffT, bool USE_RU>
 __global__ void __launch_bounds__(256)
     k_stencil_rhs(FusedArgs a, const double *__restrict__ vx, const double *__restrict__ vy,
@@ -555,135 +552,147 @@ __global__ void __launch_bounds__(256)
                   double *__restrict__ bz, double *__restrict__ bt, const double *__restrict__ stalep,
                   const double *__restrict__ kc, const double *__restrict__ facp) {
   const Geom g = a.g;
-  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t col = gid / g.ldx;
-  const int s = (int)(gid - col * g.ldx);
+  const int64_t col = xcd_block(g.ncol);   // one column per block, strip order
+  if (col < 0) return;
+  int jc, ic;
+  strip_col(col, g, jc, ic);
   const int L = g.L, ldx = g.ldx, nx = g.nx, nxm = g.nxm;
-  const int k = k_of_pos(s, L);
-  if (col >= g.ncol || k >= nx) return;
-  const int jc = (int)(col % g.nym), ic = (int)(col / g.nym);
   const int jm = wrapm(jc - 1, g.nym), jp = wrapp(jc + 1, g.nym);
   const int im = wrapm(ic - 1, g.nzm), ip = wrapp(ic + 1, g.nzm);
   const OffT cb = (OffT)ldx * sizeof(double);
   auto colb = [&](int jj, int ii) { return ((OffT)ii * (OffT)g.nym + (OffT)jj) * cb; };
   const OffT C = colb(jc, ic), CJM = colb(jm, ic), CJP = colb(jp, ic), CIM = colb(jc, im), CIP = colb(jc, ip),
              CJMIP = colb(jm, ip), CJPIM = colb(jp, im);
-  const OffT sb = (OffT)s * sizeof(double);
-  // k-1 / k+1 positions; at the column ends they point at k itself (values unused there)
-  const OffT smb = (OffT)(k >= 1 ? pos_of_k(k - 1, L) : s) * sizeof(double);
-  const OffT spb = (OffT)(k + 1 < nx ? pos_of_k(k + 1, L) : s) * sizeof(double);
-  const bool inx = k >= 1 && k <= nx - 2;   // interior rows of the nx systems
-  const bool iny = k <= nxm - 1;            // rows of the nxm systems
-  const StepScalars &c = a.sc;
-  const OffT rowb = (OffT)ldx * sizeof(double);
-  auto kcv = [&](int row) { return ldf(kc, (OffT)row * rowb + sb); };
-  // centre column, k-1 / k / k+1
-  const double vxm = ldf(vx, C + smb), vxc = ldf(vx, C + sb), vxp = ldf(vx, C + spb);
-  const double vym = ldf(vy, C + smb), vyc = ldf(vy, C + sb), vyp = ldf(vy, C + spb);
-  const double vzm = ldf(vz, C + smb), vzc = ldf(vz, C + sb), vzp = ldf(vz, C + spb);
-  const double tm = ldf(T, C + smb), tc = ldf(T, C + sb), tp = ldf(T, C + spb);
-  const double prm = ldf(pr, C + smb), prc = ldf(pr, C + sb);
-  // neighbour columns at k
-  const double vxjp = ldf(vx, CJP + sb), vxjm = ldf(vx, CJM + sb), vxip = ldf(vx, CIP + sb), vxim = ldf(vx, CIM + sb);
-  const double vyjp = ldf(vy, CJP + sb), vyjm = ldf(vy, CJM + sb), vyip = ldf(vy, CIP + sb), vyim = ldf(vy, CIM + sb);
-  const double vzjp = ldf(vz, CJP + sb), vzjm = ldf(vz, CJM + sb), vzip = ldf(vz, CIP + sb), vzim = ldf(vz, CIM + sb);
-  const double tjp = ldf(T, CJP + sb), tjm = ldf(T, CJM + sb), tip = ldf(T, CIP + sb), tim = ldf(T, CIM + sb);
-  const double prjm = ldf(pr, CJM + sb), prim = ldf(pr, CIM + sb);
-  const double vyjpim = ldf(vy, CJPIM + sb), vzjmip = ldf(vz, CJMIP + sb);
-  // neighbour columns at k-1 / k+1
-  const double vyjpm = ldf(vy, CJP + smb), vzipm = ldf(vz, CIP + smb);
-  const double vxjmp = ldf(vx, CJM + spb), vximp = ldf(vx, CIM + spb);
-  const double udx3c = kcv(KC_UDX3C), udx3m = kcv(KC_UDX3M);
-  const double amc = kcv(KC_AM3CK), acc = kcv(KC_AC3CK), apc = kcv(KC_AP3CK);
-  const double ams = kcv(KC_AM3SK), acs = kcv(KC_AC3SK), aps = kcv(KC_AP3SK);
-  const double *fX = facp + (SYS_X * FR_N) * ldx, *fY = facp + (SYS_YZ * FR_N) * ldx, *fT = facp + (SYS_T * FR_N) * ldx;
-  const OffT fSC = FR_SCALE * rowb + sb, fDL = FR_DL * rowb + sb, fDU = FR_DU * rowb + sb;
+  const double stale0 = stalep[(int64_t)ic * g.nym + jc];   // stale row of this column
+  for (int s = threadIdx.x; s < ldx; s += blockDim.x) {
+    const int k = k_of_pos(s, L);
+    if (k >= nx) continue;
+    const OffT sb = (OffT)s * sizeof(double);
+    // k-1 / k+1 positions; at the column ends they point at k itself (values unused there)
+    const OffT smb = (OffT)(k >= 1 ? pos_of_k(k - 1, L) : s) * sizeof(double);
+    const OffT spb = (OffT)(k + 1 < nx ? pos_of_k(k + 1, L) : s) * sizeof(double);
+    const bool inx = k >= 1 && k <= nx - 2;   // interior rows of the nx systems
+    const bool iny = k <= nxm - 1;            // rows of the nxm systems
+    const StepScalars &c = a.sc;
+    const OffT rowb = (OffT)ldx * sizeof(double);
+    auto kcv = [&](int row) { return ldf(kc, (OffT)row * rowb + sb); };
+    // centre column, k-1 / k / k+1
+    const double vxm = ldf(vx, C + smb), vxc = ldf(vx, C + sb), vxp = ldf(vx, C + spb);
+    const double vym = ldf(vy, C + smb), vyc = ldf(vy, C + sb), vyp = ldf(vy, C + spb);
+    const double vzm = ldf(vz, C + smb), vzc = ldf(vz, C + sb), vzp = ldf(vz, C + spb);
+    const double tm = ldf(T, C + smb), tc = ldf(T, C + sb), tp = ldf(T, C + spb);
+    const double prm = ldf(pr, C + smb), prc = ldf(pr, C + sb);
+    // neighbour columns at k
+    const double vxjp = ldf(vx, CJP + sb), vxjm = ldf(vx, CJM + sb), vxip = ldf(vx, CIP + sb), vxim = ldf(vx, CIM + sb);
+    const double vyjp = ldf(vy, CJP + sb), vyjm = ldf(vy, CJM + sb), vyip = ldf(vy, CIP + sb), vyim = ldf(vy, CIM + sb);
+    const double vzjp = ldf(vz, CJP + sb), vzjm = ldf(vz, CJM + sb), vzip = ldf(vz, CIP + sb), vzim = ldf(vz, CIM + sb);
+    const double tjp = ldf(T, CJP + sb), tjm = ldf(T, CJM + sb), tip = ldf(T, CIP + sb), tim = ldf(T, CIM + sb);
+    const double prjm = ldf(pr, CJM + sb), prim = ldf(pr, CIM + sb);
+    const double vyjpim = ldf(vy, CJPIM + sb), vzjmip = ldf(vz, CJMIP + sb);
+    // neighbour columns at k-1 / k+1
+    const double vyjpm = ldf(vy, CJP + smb), vzipm = ldf(vz, CIP + smb);
+    const double vxjmp = ldf(vx, CJM + spb), vximp = ldf(vx, CIM + spb);
+    const double udx3c = kcv(KC_UDX3C), udx3m = kcv(KC_UDX3M);
+    const double amc = kcv(KC_AM3CK), acc = kcv(KC_AC3CK), apc = kcv(KC_AP3CK);
+    const double ams = kcv(KC_AM3SK), acs = kcv(KC_AC3SK), aps = kcv(KC_AP3SK);
+    const double *fX = facp + (SYS_X * FR_N) * ldx, *fY = facp + (SYS_YZ * FR_N) * ldx, *fT = facp + (SYS_T * FR_N) * ldx;
+    const OffT fSC = FR_SCALE * rowb + sb, fDL = FR_DL * rowb + sb, fDU = FR_DU * rowb + sb;
 
-  // ---- vx (ExplicitTermsVX.F90:55-101, ImplicitAndUpdateVX.F90:47-60)
-  {
-    double hxz = (((vzip + vzipm) * (vxip + vxc)) - ((vzc + vzm) * (vxc + vxim))) * c.udz4;
-    double hxy = (((vyjp + vyjpm) * (vxjp + vxc)) - ((vyc + vym) * (vxc + vxjm))) * c.udy4;
-    double aa = vxp + vxc, bb = vxc + vxm;
-    double hxx = (aa * aa - bb * bb) * udx3c * 0.25;
-    double dzzvx = (vxim - 2.0 * vxc + vxip) * c.udzq_re;
-    double dyyvx = (vxjm - 2.0 * vxc + vxjp) * c.udyq_re;
-    double q = -(hxx + hxy + hxz) + dyyvx + dzzvx + tc;
-    double dxxvx = vxp * apc + vxc * acc + vxm * amc;
-    double dxp = (prc - prm) * (c.al * udx3c);
-    double r = USE_RU ? c.ro * ldf(rux, C + sb) : 0.0;
-    double rhs = (c.ga * q + r + c.alre * dxxvx - dxp) * c.dt;
-    double vin = rhs * ldf(fX, fSC) + ((ldf(fX, fDL) * vxm + vxc) + ldf(fX, fDU) * vxp);
-    double vbd = (k == 0 ? stalep[col] : 0.0) + vxc;
-    if (inx) stf(rux, C + sb, q);
-    stf(bx, C + sb, inx ? vin : vbd);
-  }
-  // ---- vy (ExplicitTermsVY.F90:52-98, ImplicitAndUpdateVY.F90:52-68); kmv/kpv clamps
-  {
-    const double vymm = k == 0 ? vyc : vym, vypp = k == nxm - 1 ? vyc : vyp;
-    double hyx = ((vxp + vxjmp) * (vypp + vyc) - (vxc + vxjm) * (vyc + vymm)) * udx3m * 0.25;
-    double yp = vyjp + vyc, ym = vyjm + vyc;
-    double hyy = (yp * yp - ym * ym) * c.udy4;
-    double hyz = ((vyip + vyc) * (vzip + vzjmip) - (vyc + vyim) * (vzc + vzjm)) * c.udz4;
-    double dyyvy = (vyjp - 2.0 * vyc + vyjm) * c.udyq_re;
-    double dzzvy = (vyip - 2.0 * vyc + vyim) * c.udzq_re;
-    double h = -(hyx + hyy + hyz) + dyyvy + dzzvy;
-    double dxxvy = vypp * aps + vyc * acs + vymm * ams;
-    double dyp = (prc - prjm) * c.udy_al;
-    double r = USE_RU ? c.ro * ldf(ruy, C + sb) : 0.0;
-    double rhs = (c.ga * h + r + c.alre * dxxvy - dyp) * c.dt;
-    double dl = ldf(fY, fDL), du = ldf(fY, fDU);
-    double vin = rhs * ldf(fY, fSC) + ((dl * vym + vyc) + du * vyp);
-    if (iny) {
-      stf(ruy, C + sb, h);
-      stf(by, C + sb, vin);
+    // ---- vx (ExplicitTermsVX.F90:55-101, ImplicitAndUpdateVX.F90:47-60)
+    {
+      double hxz = (((vzip + vzipm) * (vxip + vxc)) - ((vzc + vzm) * (vxc + vxim))) * c.udz4;
+      double hxy = (((vyjp + vyjpm) * (vxjp + vxc)) - ((vyc + vym) * (vxc + vxjm))) * c.udy4;
+      double aa = vxp + vxc, bb = vxc + vxm;
+      double hxx = (aa * aa - bb * bb) * udx3c * 0.25;
+      double dzzvx = (vxim - 2.0 * vxc + vxip) * c.udzq_re;
+      double dyyvx = (vxjm - 2.0 * vxc + vxjp) * c.udyq_re;
+      double q = -(hxx + hxy + hxz) + dyyvx + dzzvx + tc;
+      double dxxvx = vxp * apc + vxc * acc + vxm * amc;
+      double dxp = (prc - prm) * (c.al * udx3c);
+      double r = USE_RU ? c.ro * ldf(rux, C + sb) : 0.0;
+      double rhs = (c.ga * q + r + c.alre * dxxvx - dxp) * c.dt;
+      double vin = rhs * ldf(fX, fSC) + ((ldf(fX, fDL) * vxm + vxc) + ldf(fX, fDU) * vxp);
+      double vbd = (k == 0 ? stale0 : 0.0) + vxc;
+      stf(rux, C + sb, inx ? q : 0.0);
+      stf(bx, C + sb, inx ? vin : vbd);
     }
-  }
-  // ---- vz (ExplicitTermsVZ.F90:56-101, ImplicitAndUpdateVZ.F90:48-64)
-  {
-    const double vzmm = k == 0 ? vzc : vzm, vzpp = k == nxm - 1 ? vzc : vzp;
-    double zp = vzip + vzc, zm = vzim + vzc;
-    double hzz = (zp * zp - zm * zm) * c.udz4;
-    double hzy = ((vyjp + vyjpim) * (vzjp + vzc) - (vyc + vyim) * (vzc + vzjm)) * c.udy4;
-    double hzx = ((vxp + vximp) * (vzpp + vzc) - (vxc + vxim) * (vzc + vzmm)) * udx3m * 0.25;
-    double dzzvz = (vzip - 2.0 * vzc + vzim) * c.udzq_re;
-    double dyyvz = (vzjp - 2.0 * vzc + vzjm) * c.udyq_re;
-    double h = -(hzx + hzy + hzz) + dyyvz + dzzvz;
-    double dxxvz = vzpp * aps + vzc * acs + vzmm * ams;
-    double dzp = (prc - prim) * c.dz * c.al;
-    double r = USE_RU ? c.ro * ldf(ruz, C + sb) : 0.0;
-    double rhs = (c.ga * h + r + c.alre * dxxvz - dzp) * c.dt;
-    double dl = ldf(fY, fDL), du = ldf(fY, fDU);
-    double vin = rhs * ldf(fY, fSC) + ((dl * vzm + vzc) + du * vzp);
-    if (iny) {
-      stf(ruz, C + sb, h);
+    // ---- vy (ExplicitTermsVY.F90:52-98, ImplicitAndUpdateVY.F90:52-68); kmv/kpv clamps
+    {
+      const double vymm = k == 0 ? vyc : vym, vypp = k == nxm - 1 ? vyc : vyp;
+      double hyx = ((vxp + vxjmp) * (vypp + vyc) - (vxc + vxjm) * (vyc + vymm)) * udx3m * 0.25;
+      double yp = vyjp + vyc, ym = vyjm + vyc;
+      double hyy = (yp * yp - ym * ym) * c.udy4;
+      double hyz = ((vyip + vyc) * (vzip + vzjmip) - (vyc + vyim) * (vzc + vzjm)) * c.udz4;
+      double dyyvy = (vyjp - 2.0 * vyc + vyjm) * c.udyq_re;
+      double dzzvy = (vyip - 2.0 * vyc + vyim) * c.udzq_re;
+      double h = -(hyx + hyy + hyz) + dyyvy + dzzvy;
+      double dxxvy = vypp * aps + vyc * acs + vymm * ams;
+      double dyp = (prc - prjm) * c.udy_al;
+      double r = USE_RU ? c.ro * ldf(ruy, C + sb) : 0.0;
+      double rhs = (c.ga * h + r + c.alre * dxxvy - dyp) * c.dt;
+      double dl = ldf(fY, fDL), du = ldf(fY, fDU);
+      double vin = rhs * ldf(fY, fSC) + ((dl * vym + vyc) + du * vyp);
+      stf(ruy, C + sb, iny ? h : 0.0);
+      stf(by, C + sb, vin);   // row nx-1 is outside the nxm system and never read
+    }
+    // ---- vz (ExplicitTermsVZ.F90:56-101, ImplicitAndUpdateVZ.F90:48-64)
+    {
+      const double vzmm = k == 0 ? vzc : vzm, vzpp = k == nxm - 1 ? vzc : vzp;
+      double zp = vzip + vzc, zm = vzim + vzc;
+      double hzz = (zp * zp - zm * zm) * c.udz4;
+      double hzy = ((vyjp + vyjpim) * (vzjp + vzc) - (vyc + vyim) * (vzc + vzjm)) * c.udy4;
+      double hzx = ((vxp + vximp) * (vzpp + vzc) - (vxc + vxim) * (vzc + vzmm)) * udx3m * 0.25;
+      double dzzvz = (vzip - 2.0 * vzc + vzim) * c.udzq_re;
+      double dyyvz = (vzjp - 2.0 * vzc + vzjm) * c.udyq_re;
+      double h = -(hzx + hzy + hzz) + dyyvz + dzzvz;
+      double dxxvz = vzpp * aps + vzc * acs + vzmm * ams;
+      double dzp = (prc - prim) * c.dz * c.al;
+      double r = USE_RU ? c.ro * ldf(ruz, C + sb) : 0.0;
+      double rhs = (c.ga * h + r + c.alre * dxxvz - dzp) * c.dt;
+      double dl = ldf(fY, fDL), du = ldf(fY, fDU);
+      double vin = rhs * ldf(fY, fSC) + ((dl * vzm + vzc) + du * vzp);
+      stf(ruz, C + sb, iny ? h : 0.0);
       stf(bz, C + sb, vin);
     }
-  }
-  // ---- temperature (ExplicitTermsTemp.F90:52-93, ImplicitAndUpdateTemp.F90:41-54)
-  {
-    double htz = ((vzipm + vzip) * (tip + tc) - (vzm + vzc) * (tc + tim)) * c.udz4;
-    double hty = ((vyjp + vyjpm) * (tjp + tc) - (vyc + vym) * (tc + tjm)) * c.udy4;
-    double htx = ((vxp + vxc) * (tp + tc) - (vxc + vxm) * (tc + tm)) * udx3c * 0.25;
-    double dzzt = (tip - 2.0 * tc + tim) * c.udzq_pe;
-    double dyyt = (tjp - 2.0 * tc + tjm) * c.udyq_pe;
-    double h = -(htx + hty + htz) + dyyt + dzzt;
-    double dxxt = tp * apc + tc * acc + tm * amc;
-    double r = USE_RU ? c.ro * ldf(rut, C + sb) : 0.0;
-    double rhs = (c.ga * h + r + c.alpec * dxxt) * c.dt;
-    double vin = rhs * ldf(fT, fSC) + ((ldf(fT, fDL) * tm + tc) + ldf(fT, fDU) * tp);
-    if (inx) stf(rut, C + sb, h);
-    stf(bt, C + sb, inx ? vin : tc);   // row 1 gets the vz increment added by the solve pass
+    // ---- temperature (ExplicitTermsTemp.F90:52-93, ImplicitAndUpdateTemp.F90:41-54)
+    {
+      double htz = ((vzipm + vzip) * (tip + tc) - (vzm + vzc) * (tc + tim)) * c.udz4;
+      double hty = ((vyjp + vyjpm) * (tjp + tc) - (vyc + vym) * (tc + tjm)) * c.udy4;
+      double htx = ((vxp + vxc) * (tp + tc) - (vxc + vxm) * (tc + tm)) * udx3c * 0.25;
+      double dzzt = (tip - 2.0 * tc + tim) * c.udzq_pe;
+      double dyyt = (tjp - 2.0 * tc + tjm) * c.udyq_pe;
+      double h = -(htx + hty + htz) + dyyt + dzzt;
+      double dxxt = tp * apc + tc * acc + tm * amc;
+      double r = USE_RU ? c.ro * ldf(rut, C + sb) : 0.0;
+      double rhs = (c.ga * h + r + c.alpec * dxxt) * c.dt;
+      double vin = rhs * ldf(fT, fSC) + ((ldf(fT, fDL) * tm + tc) + ldf(fT, fDU) * tp);
+      stf(rut, C + sb, inx ? h : 0.0);
+      stf(bt, C + sb, inx ? vin : tc);   // row 1 gets the vz increment added by the solve pass
+    }
   }
 }
 
-// Loads lane p's chunk of b' into registers, solves, returns cin (x = b + H cin).
+// Lane p's chunk of one column's b' into registers (zeros past cnt).  The
+// loads are unconditional (slot index clamped into the column's L rows) so
+// that all LM of them issue back to back; the select discards the extras.
 template <int LM, typename OffT>
-__device__ __forceinline__ double solve_chunk_from(double (&b)[LM], const double *__restrict__ src, OffT C,
-                                                   const double *fac, int ldx, int cnt, int p) {
-  const double *fl = fac + FR_L * ldx, *fr = fac + FR_RINV * ldx, *fe = fac + FR_E * ldx;
-  const double *fg = fac + FR_G * ldx, *fh = fac + FR_H * ldx;
+__device__ __forceinline__ void load_chunk(double (&b)[LM], const double *__restrict__ src, OffT C, int cnt,
+                                           int L, int p) {
 #pragma unroll
-  for (int t = 0; t < LM; t++) b[t] = t < cnt ? ldf(src, C + (OffT)(t * P + p) * sizeof(double)) : 0.0;
+  for (int t = 0; t < LM; t++) {
+    const double v = ldf(src, C + (OffT)(min(t, L - 1) * P + p) * sizeof(double));
+    b[t] = t < cnt ? v : 0.0;
+  }
+}
+
+// Solves the chunk held in b in place; returns cin, with the solution
+// x = b + H cin.  `rows` holds the system's factor rows L, RINV, E, G, H
+// (row r at rows + r*ldx), staged in LDS by the caller.
+enum { RW_L = 0, RW_RINV, RW_E, RW_G, RW_H, RW_N };
+template <int LM>
+__device__ __forceinline__ double solve_chunk(double (&b)[LM], const double *rows, int ldx, int cnt, int p) {
+  const double *fl = rows + RW_L * ldx, *fr = rows + RW_RINV * ldx, *fe = rows + RW_E * ldx;
+  const double *fg = rows + RW_G * ldx, *fh = rows + RW_H * ldx;
   double acc = 0.0;
 #pragma unroll
   for (int t = 0; t < LM; t++) {
@@ -706,79 +715,100 @@ __device__ __forceinline__ double solve_chunk_from(double (&b)[LM], const double
   return scan_down(cnt > 0 ? fh[p] : 1.0, cnt > 0 ? acc : 0.0, p);
 }
 
+constexpr int SOLVE_WG = 512;                 // 8 waves, 32 columns per block
+constexpr int SOLVE_COLS = SOLVE_WG / P;
+
+// LDS bytes of k_column_solve: rows L..H of the three systems.
+static size_t solve_lds(const Geom &g) { return sizeof(double) * SYS_N * RW_N * (size_t)g.ldx; }
+
 template <int LM, typename OffT>
-__global__ void __launch_bounds__(256)
-    k_column_solve(FusedArgs a, const double *__restrict__ vz, const double *__restrict__ T,
-                   double *__restrict__ bx, double *__restrict__ by, double *__restrict__ bz,
-                   double *__restrict__ bt, double *__restrict__ vxn, double *__restrict__ vyn,
-                   double *__restrict__ vzn, double *__restrict__ tn, double *__restrict__ stalep,
-                   const double *__restrict__ facp) {
+__global__ void __launch_bounds__(SOLVE_WG)
+    k_column_solve(FusedArgs a, const double *__restrict__ vz, const double *__restrict__ bx,
+                   const double *__restrict__ by, const double *__restrict__ bz, const double *__restrict__ bt,
+                   double *__restrict__ vxn, double *__restrict__ vyn, double *__restrict__ vzn,
+                   double *__restrict__ tn, double *__restrict__ stalep, const double *__restrict__ facp) {
+  extern __shared__ double rows[];   // [SYS_N][RW_N][ldx]
   const Geom g = a.g;
-  const int64_t blk = xcd_block((g.ncol + COLS_PER_WG - 1) / COLS_PER_WG);
+  const int64_t blk = xcd_block((g.ncol + SOLVE_COLS - 1) / SOLVE_COLS);
   if (blk < 0) return;
+  const int L = g.L, ldx = g.ldx, nx = g.nx, nxm = g.nxm;
+  // the factor rows are the same for every column: stage them once per block
+  for (int idx = threadIdx.x; idx < SYS_N * RW_N * ldx; idx += SOLVE_WG) {
+    const int sr = idx / ldx, s = idx - sr * ldx;
+    const int sys = sr / RW_N, r = sr - sys * RW_N;
+    rows[idx] = facp[(sys * FR_N + FR_L + r) * ldx + s];
+  }
+  __syncthreads();
   const int p = threadIdx.x % P;
-  const int64_t col = blk * COLS_PER_WG + threadIdx.x / P;
+  const int64_t col = blk * SOLVE_COLS + threadIdx.x / P;
   const bool live = col < g.ncol;
   const int64_t colc = live ? col : 0;
   const OffT C = (OffT)colc * (OffT)g.ldx * sizeof(double);
-  const int L = g.L, ldx = g.ldx, nx = g.nx, nxm = g.nxm;
   const int k0 = p * L;
   const int cntx = live ? max(0, min(L, nx - k0)) : 0;
   const int cntm = live ? max(0, min(L, nxm - k0)) : 0;
+  const double *rX = rows + (SYS_X * RW_N) * ldx, *rY = rows + (SYS_YZ * RW_N) * ldx,
+               *rT = rows + (SYS_T * RW_N) * ldx;
   double b[LM];
+  load_chunk<LM, OffT>(b, bx, C, cntx, L, p);
   {  // vx
-    const double *fac = facp + (SYS_X * FR_N) * ldx, *fh = fac + FR_H * ldx;
-    const double cin = solve_chunk_from<LM, OffT>(b, bx, C, fac, ldx, cntx, p);
+    const double *fh = rX + RW_H * ldx;
+    const double cin = solve_chunk<LM>(b, rX, ldx, cntx, p);
+    if (live)
 #pragma unroll
-    for (int t = 0; t < LM; t++)
-      if (t < cntx) {
-        const int k = k0 + t, s = t * P + p;
-        stf(vxn, C + (OffT)s * sizeof(double), (k >= 1 && k <= nx - 2) ? b[t] + fh[s] * cin : 0.0);
-      }
+      for (int t = 0; t < LM; t++)
+        if (t < L) {   // slots of k >= nx (t >= cntx) are written 0
+          const int k = k0 + t, s = t * P + p;
+          stf(vxn, C + (OffT)s * sizeof(double), (k >= 1 && k <= nx - 2) ? b[t] + fh[s] * cin : 0.0);
+        }
   }
+  load_chunk<LM, OffT>(b, by, C, cntm, L, p);
   {  // vy
-    const double *fac = facp + (SYS_YZ * FR_N) * ldx, *fh = fac + FR_H * ldx;
-    const double cin = solve_chunk_from<LM, OffT>(b, by, C, fac, ldx, cntm, p);
+    const double *fh = rY + RW_H * ldx;
+    const double cin = solve_chunk<LM>(b, rY, ldx, cntm, p);
+    if (live)
 #pragma unroll
-    for (int t = 0; t < LM; t++)
-      if (t < cntx) {
-        const int k = k0 + t, s = t * P + p;
-        stf(vyn, C + (OffT)s * sizeof(double), k < nxm ? b[t] + fh[s] * cin : 0.0);
-      }
+      for (int t = 0; t < LM; t++)
+        if (t < L) {
+          const int k = k0 + t, s = t * P + p;
+          stf(vyn, C + (OffT)s * sizeof(double), k < nxm ? b[t] + fh[s] * cin : 0.0);
+        }
   }
+  load_chunk<LM, OffT>(b, bz, C, cntm, L, p);
+  const double vz0 = (live && p == 0) ? ldf(vz, C) : 0.0;   // position of k = 0 is slot 0
   double dvz0 = 0.0;
   {  // vz
-    const double *fac = facp + (SYS_YZ * FR_N) * ldx, *fh = fac + FR_H * ldx;
-    const double cin = solve_chunk_from<LM, OffT>(b, bz, C, fac, ldx, cntm, p);
-    const double vz0 = (live && p == 0) ? ldf(vz, C) : 0.0;   // position of k = 0 is slot 0
+    const double *fh = rY + RW_H * ldx;
+    const double cin = solve_chunk<LM>(b, rY, ldx, cntm, p);
+    dvz0 = b[0] + fh[p] * cin - vz0;   // used from lane p = 0 (k = 0) only
+    if (live)
 #pragma unroll
-    for (int t = 0; t < LM; t++)
-      if (t < cntx) {
-        const int k = k0 + t, s = t * P + p;
-        const double y = k < nxm ? b[t] + fh[s] * cin : 0.0;
-        if (t == 0 && k == 0) dvz0 = y - vz0;
-        stf(vzn, C + (OffT)s * sizeof(double), y);
-      }
+      for (int t = 0; t < LM; t++)
+        if (t < L) {
+          const int k = k0 + t, s = t * P + p;
+          stf(vzn, C + (OffT)s * sizeof(double), k < nxm ? b[t] + fh[s] * cin : 0.0);
+        }
   }
   dvz0 = __shfl(dvz0, 0, P);
+  load_chunk<LM, OffT>(b, bt, C, cntx, L, p);
   if (live && p == 0) {
     stalep[colc] = dvz0;
-    stf(bt, C, ldf(bt, C) + dvz0);   // temperature row 1: rhs(1) = vz increment
+    b[0] += dvz0;   // temperature row 1: rhs(1) = vz increment (lane 0 holds k = 0)
   }
   {  // temperature
-    const double *fac = facp + (SYS_T * FR_N) * ldx, *fh = fac + FR_H * ldx;
-    const double cin = solve_chunk_from<LM, OffT>(b, bt, C, fac, ldx, cntx, p);
+    const double *fh = rT + RW_H * ldx;
+    const double cin = solve_chunk<LM>(b, rT, ldx, cntx, p);
+    if (live)
 #pragma unroll
-    for (int t = 0; t < LM; t++)
-      if (t < cntx) {
-        const int k = k0 + t, s = t * P + p;
-        double y = b[t] + fh[s] * cin;
-        if (k == 0) y = a.sc.tempbp;
-        if (k == nx - 1) y = a.sc.temptp;
-        stf(tn, C + (OffT)s * sizeof(double), y);
-      }
+      for (int t = 0; t < LM; t++)
+        if (t < L) {
+          const int k = k0 + t, s = t * P + p;
+          double y = b[t] + fh[s] * cin;
+          y = k == 0 ? a.sc.tempbp : y;
+          y = k == nx - 1 ? a.sc.temptp : y;
+          stf(tn, C + (OffT)s * sizeof(double), k < nx ? y : 0.0);
+        }
   }
-  (void)T;
 }
 
 // ---------------------------------------------------------------- per-routine kernels
@@ -919,35 +949,42 @@ __global__ void k_divergence(RoutineArgs a) {
 }
 
 // CorrectVelocity.F90:28-45 and CorrectPressure.F90:32-52 from dph (= dphhalo).
-// which: 1 velocity, 2 pressure, 3 both (the fused path).
-__global__ void k_correct(CorrectArgs a, int which) {
+// which: 1 velocity, 2 pressure, 3 both (the fused path).  One block per
+// column in strip order (XCD-remapped): the y/z neighbour columns of dph were
+// read by nearby blocks of the same XCD.
+__global__ void __launch_bounds__(256) k_correct(CorrectArgs a, int which) {
   const Geom g = a.g;
-  PointIdx q = point_idx(g);
-  if (!q.live || q.k > g.nxm - 1) return;
-  const int j = (int)(q.col % g.nym), i = (int)(q.col / g.nym);
+  const int64_t col = xcd_block(g.ncol);
+  if (col < 0) return;
+  int j, i;
+  strip_col(col, g, j, i);
   const int jm = wrapm(j - 1, g.nym), jp = wrapp(j + 1, g.nym);
   const int im = wrapm(i - 1, g.nzm), ip = wrapp(i + 1, g.nzm);
-  const int64_t C = col_off(j, i, g);
-  const int s = q.s, k = q.k, ldx = g.ldx;
-  const int skm = k == 0 ? s : pos_of_k(k - 1, g.L);
-  const int skp = k == g.nxm - 1 ? s : pos_of_k(k + 1, g.L);
-  const double *dph = a.dph;
-  const double d = dph[C + s];
+  const int64_t C = col_off(j, i, g), CJM = col_off(jm, i, g), CJP = col_off(jp, i, g), CIM = col_off(j, im, g),
+                CIP = col_off(j, ip, g);
+  const int ldx = g.ldx;
+  const double *__restrict__ dph = a.dph;
   const StepScalars &c = a.sc;
-  if (which & 1) {
-    double usukm = c.al * c.dt * a.kc[KC_UDX3C * ldx + s];
-    double udy = c.al * c.dt * c.dy, udz = c.al * c.dt * c.dz;
-    a.vx[C + s] = a.vx[C + s] - (d - dph[C + skm]) * usukm;
-    a.vy[C + s] = a.vy[C + s] - (d - dph[col_off(jm, i, g) + s]) * udy;
-    a.vz[C + s] = a.vz[C + s] - (d - dph[col_off(j, im, g) + s]) * udz;
-  }
-  if (which & 2) {
-    double be = c.al * c.beta;
-    double amm = a.kc[KC_AMPHK * ldx + s], acc = a.kc[KC_ACPHK * ldx + s], app = a.kc[KC_APPHK * ldx + s];
-    a.pr[C + s] = a.pr[C + s] + d -
-                  be * ((dph[col_off(j, ip, g) + s] - 2.0 * d + dph[col_off(j, im, g) + s]) * c.dzq +
-                        (dph[col_off(jp, i, g) + s] - 2.0 * d + dph[col_off(jm, i, g) + s]) * c.dyq +
-                        (dph[C + skp] * app + d * acc + dph[C + skm] * amm));
+  for (int s = threadIdx.x; s < ldx; s += blockDim.x) {
+    const int k = k_of_pos(s, g.L);
+    if (k > g.nxm - 1) continue;
+    const int skm = k == 0 ? s : pos_of_k(k - 1, g.L);
+    const int skp = k == g.nxm - 1 ? s : pos_of_k(k + 1, g.L);
+    const double d = dph[C + s], dkm = dph[C + skm], djm = dph[CJM + s], dim_ = dph[CIM + s];
+    if (which & 1) {
+      double usukm = c.al * c.dt * a.kc[KC_UDX3C * ldx + s];
+      double udy = c.al * c.dt * c.dy, udz = c.al * c.dt * c.dz;
+      a.vx[C + s] = a.vx[C + s] - (d - dkm) * usukm;
+      a.vy[C + s] = a.vy[C + s] - (d - djm) * udy;
+      a.vz[C + s] = a.vz[C + s] - (d - dim_) * udz;
+    }
+    if (which & 2) {
+      double be = c.al * c.beta;
+      double amm = a.kc[KC_AMPHK * ldx + s], acc = a.kc[KC_ACPHK * ldx + s], app = a.kc[KC_APPHK * ldx + s];
+      a.pr[C + s] = a.pr[C + s] + d -
+                    be * ((dph[CIP + s] - 2.0 * d + dim_) * c.dzq + (dph[CJP + s] - 2.0 * d + djm) * c.dyq +
+                          (dph[C + skp] * app + d * acc + dkm * amm));
+    }
   }
 }
 
@@ -976,6 +1013,15 @@ hipError_t column_configure_lds(size_t bytes) {
   AFID_LM_LIST(SETA)
 #undef SETA
 #undef SETA1
+#define SETB(V)                                                                                          \
+  e = hipFuncSetAttribute((const void *)k_column_solve<V, uint32_t>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                          (int)bytes);                                                                   \
+  if (e != hipSuccess) return e;                                                                         \
+  e = hipFuncSetAttribute((const void *)k_column_solve<V, uint64_t>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                          (int)bytes);                                                                   \
+  if (e != hipSuccess) return e;
+  AFID_LM_LIST(SETB)
+#undef SETB
   return hipSuccess;
 }
 
@@ -1013,7 +1059,7 @@ hipError_t launch_fused_update(const FusedArgs &a, hipStream_t st) {
 
 hipError_t launch_stencil_rhs(const FusedArgs &a, hipStream_t st) {
   const int64_t n = a.g.ncol * a.g.ldx;
-  const dim3 grid((unsigned)((n + 255) / 256));
+  const dim3 grid((unsigned)(((a.g.ncol + 7) / 8) * 8));
   const bool small = (uint64_t)n * sizeof(double) < (1ull << 32);
   const bool ru = a.sc.use_ru != 0;
 #define SR(O, R)                                                                                         \
@@ -1029,15 +1075,17 @@ hipError_t launch_stencil_rhs(const FusedArgs &a, hipStream_t st) {
 
 hipError_t launch_column_solve(const FusedArgs &a, hipStream_t st) {
   const bool small = (uint64_t)a.g.ncol * a.g.ldx * sizeof(double) < (1ull << 32);
-  const dim3 grid = column_grid(a.g);
+  const int64_t nblk = (a.g.ncol + SOLVE_COLS - 1) / SOLVE_COLS;
+  const dim3 grid((unsigned)(((nblk + 7) / 8) * 8));
+  const size_t shm = solve_lds(a.g);
   switch (pick_lm(a.g.L)) {
 #define CASE(V)                                                                                        \
   case V:                                                                                              \
     if (small)                                                                                         \
-      hipLaunchKernelGGL((k_column_solve<V, uint32_t>), grid, dim3(256), 0, st, a, a.f.vz, a.f.temp, a.bx, \
+      hipLaunchKernelGGL((k_column_solve<V, uint32_t>), grid, dim3(SOLVE_WG), shm, st, a, a.f.vz, a.bx, \
                          a.by, a.bz, a.bt, a.vxn, a.vyn, a.vzn, a.tn, a.stale, a.fac);                  \
     else                                                                                               \
-      hipLaunchKernelGGL((k_column_solve<V, uint64_t>), grid, dim3(256), 0, st, a, a.f.vz, a.f.temp, a.bx, \
+      hipLaunchKernelGGL((k_column_solve<V, uint64_t>), grid, dim3(SOLVE_WG), shm, st, a, a.f.vz, a.bx, \
                          a.by, a.bz, a.bt, a.vxn, a.vyn, a.vzn, a.tn, a.stale, a.fac);                  \
     return hipGetLastError();
     AFID_LM_LIST(CASE)
@@ -1070,7 +1118,7 @@ hipError_t launch_divergence(const RoutineArgs &a, hipStream_t st) {
   return hipGetLastError();
 }
 hipError_t launch_correct(const CorrectArgs &a, int which, hipStream_t st) {
-  hipLaunchKernelGGL(k_correct, point_grid(a.g), dim3(256), 0, st, a, which);
+  hipLaunchKernelGGL(k_correct, dim3((unsigned)(((a.g.ncol + 7) / 8) * 8)), dim3(256), 0, st, a, which);
   return hipGetLastError();
 }
 

@@ -30,11 +30,21 @@ __device__ __forceinline__ double2 twid(const double2 *tw, int idx, double sign)
   return t;
 }
 
+// LDS writes of this wave are complete and visible to its own later reads.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // In-place DIT stages over `nlines` lines of length pl.n stored contiguously
-// in LDS, input already in digit-reversed order.  sign = -1 forward, +1 backward.
+// in LDS from `lines`, input already in digit-reversed order, computed by ONE
+// wavefront (lane = 0..63): the stages synchronise within the wave only, so
+// the waves of a block progress independently.  sign = -1 forward, +1 backward.
 // Stage with radix r and span m combines, for every block b and offset u < m,
 // the r points b*m*r + u + v*m (v < r) after twiddling point v by W_{mr}^{uv}.
-__device__ __forceinline__ void lds_fft(double2 *lds, int nlines, const FFTPlan &pl, double sign) {
+__device__ __forceinline__ void lds_fft_wave(double2 *lines, int nlines, const FFTPlan &pl, double sign,
+                                             int lane) {
   const int n = pl.n;
   const double2 *tw = pl.tw;
   int m = 1;
@@ -42,78 +52,95 @@ __device__ __forceinline__ void lds_fft(double2 *lds, int nlines, const FFTPlan 
     const int r = pl.f[st];
     const int M = m * r;
     const int nb = n / r;
-    const int total = nlines * nb;
     const int tws = n / M;
-    for (int id = threadIdx.x; id < total; id += blockDim.x) {
-      const int q = id / nb, w = id - q * nb;
-      const int bb = w / m, u = w - bb * m;
-      double2 *base = lds + q * n + bb * M + u;
-      const int t1i = u * tws;
-      if (r == 2) {
-        double2 x0 = base[0], x1 = cmul(base[m], twid(tw, t1i, sign));
-        base[0] = cadd(x0, x1);
-        base[m] = csub(x0, x1);
-      } else if (r == 4) {
-        double2 x0 = base[0];
-        double2 x1 = cmul(base[m], twid(tw, t1i, sign));
-        double2 x2 = cmul(base[2 * m], twid(tw, 2 * t1i, sign));
-        double2 x3 = cmul(base[3 * m], twid(tw, 3 * t1i, sign));
-        double2 a0 = cadd(x0, x2), a1 = csub(x0, x2);
-        double2 b0 = cadd(x1, x3), b1 = mul_si(csub(x1, x3), sign);
-        base[0] = cadd(a0, b0);
-        base[m] = cadd(a1, b1);
-        base[2 * m] = csub(a0, b0);
-        base[3 * m] = csub(a1, b1);
-      } else if (r == 3) {
-        const double s3 = 0.86602540378443864676372317075294;  // sin(2 pi/3)
-        double2 x0 = base[0];
-        double2 x1 = cmul(base[m], twid(tw, t1i, sign));
-        double2 x2 = cmul(base[2 * m], twid(tw, 2 * t1i, sign));
-        double2 t1 = cadd(x1, x2);
-        double2 t2 = mul_si(csub(x1, x2), sign * s3);
-        double2 a = make_double2(x0.x - 0.5 * t1.x, x0.y - 0.5 * t1.y);
-        base[0] = cadd(x0, t1);
-        base[m] = cadd(a, t2);
-        base[2 * m] = csub(a, t2);
-      } else {  // r == 5
-        const double c1 = 0.30901699437494742410229341718282;   // cos(2 pi/5)
-        const double c2 = -0.80901699437494742410229341718282;  // cos(4 pi/5)
-        const double s1 = 0.95105651629515357211643933337938;   // sin(2 pi/5)
-        const double s2 = 0.58778525229247312916870595463907;   // sin(4 pi/5)
-        double2 x0 = base[0];
-        double2 x1 = cmul(base[m], twid(tw, t1i, sign));
-        double2 x2 = cmul(base[2 * m], twid(tw, 2 * t1i, sign));
-        double2 x3 = cmul(base[3 * m], twid(tw, 3 * t1i, sign));
-        double2 x4 = cmul(base[4 * m], twid(tw, 4 * t1i, sign));
-        double2 t1 = cadd(x1, x4), t2 = cadd(x2, x3);
-        double2 t3 = csub(x1, x4), t4 = csub(x2, x3);
-        double2 a1 = make_double2(x0.x + c1 * t1.x + c2 * t2.x, x0.y + c1 * t1.y + c2 * t2.y);
-        double2 a2 = make_double2(x0.x + c2 * t1.x + c1 * t2.x, x0.y + c2 * t1.y + c1 * t2.y);
-        double2 b1 = mul_si(make_double2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y), sign);
-        double2 b2 = mul_si(make_double2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y), sign);
-        base[0] = cadd(x0, cadd(t1, t2));
-        base[m] = cadd(a1, b1);
-        base[2 * m] = cadd(a2, b2);
-        base[3 * m] = csub(a2, b2);
-        base[4 * m] = csub(a1, b1);
+    const unsigned mdiv = pl.mdiv[st];
+    for (int q = 0; q < nlines; q++) {
+      double2 *line = lines + q * n;
+      for (int w = lane; w < nb; w += 64) {
+        const int bb = m == 1 ? w : (int)__umulhi((unsigned)w, mdiv);
+        const int u = w - bb * m;
+        double2 *base = line + bb * M + u;
+        const int t1i = u * tws;
+        if (r == 2) {
+          double2 x0 = base[0], x1 = cmul(base[m], twid(tw, t1i, sign));
+          base[0] = cadd(x0, x1);
+          base[m] = csub(x0, x1);
+        } else if (r == 4) {
+          double2 x0 = base[0];
+          double2 x1 = cmul(base[m], twid(tw, t1i, sign));
+          double2 x2 = cmul(base[2 * m], twid(tw, 2 * t1i, sign));
+          double2 x3 = cmul(base[3 * m], twid(tw, 3 * t1i, sign));
+          double2 a0 = cadd(x0, x2), a1 = csub(x0, x2);
+          double2 b0 = cadd(x1, x3), b1 = mul_si(csub(x1, x3), sign);
+          base[0] = cadd(a0, b0);
+          base[m] = cadd(a1, b1);
+          base[2 * m] = csub(a0, b0);
+          base[3 * m] = csub(a1, b1);
+        } else if (r == 3) {
+          const double s3 = 0.86602540378443864676372317075294;  // sin(2 pi/3)
+          double2 x0 = base[0];
+          double2 x1 = cmul(base[m], twid(tw, t1i, sign));
+          double2 x2 = cmul(base[2 * m], twid(tw, 2 * t1i, sign));
+          double2 t1 = cadd(x1, x2);
+          double2 t2 = mul_si(csub(x1, x2), sign * s3);
+          double2 a = make_double2(x0.x - 0.5 * t1.x, x0.y - 0.5 * t1.y);
+          base[0] = cadd(x0, t1);
+          base[m] = cadd(a, t2);
+          base[2 * m] = csub(a, t2);
+        } else {  // r == 5
+          const double c1 = 0.30901699437494742410229341718282;   // cos(2 pi/5)
+          const double c2 = -0.80901699437494742410229341718282;  // cos(4 pi/5)
+          const double s1 = 0.95105651629515357211643933337938;   // sin(2 pi/5)
+          const double s2 = 0.58778525229247312916870595463907;   // sin(4 pi/5)
+          double2 x0 = base[0];
+          double2 x1 = cmul(base[m], twid(tw, t1i, sign));
+          double2 x2 = cmul(base[2 * m], twid(tw, 2 * t1i, sign));
+          double2 x3 = cmul(base[3 * m], twid(tw, 3 * t1i, sign));
+          double2 x4 = cmul(base[4 * m], twid(tw, 4 * t1i, sign));
+          double2 t1 = cadd(x1, x4), t2 = cadd(x2, x3);
+          double2 t3 = csub(x1, x4), t4 = csub(x2, x3);
+          double2 a1 = make_double2(x0.x + c1 * t1.x + c2 * t2.x, x0.y + c1 * t1.y + c2 * t2.y);
+          double2 a2 = make_double2(x0.x + c2 * t1.x + c1 * t2.x, x0.y + c2 * t1.y + c1 * t2.y);
+          double2 b1 = mul_si(make_double2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y), sign);
+          double2 b2 = mul_si(make_double2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y), sign);
+          base[0] = cadd(x0, cadd(t1, t2));
+          base[m] = cadd(a1, b1);
+          base[2 * m] = cadd(a2, b2);
+          base[3 * m] = csub(a2, b2);
+          base[4 * m] = csub(a1, b1);
+        }
       }
     }
-    __syncthreads();
+    wave_lds_sync();
     m = M;
   }
+}
+
+// Block index of a 1-D launch of `total` logical tiles (grid padded to a
+// multiple of 8), remapped so that each XCD (blocks are dealt to the 8 XCDs
+// round-robin) works on one contiguous range of tiles: neighbouring tiles share
+// cache lines and stay in one L2.  Returns -1 for padding blocks.
+__device__ __forceinline__ int64_t fft_tile(int64_t total) {
+  const int64_t b = blockIdx.x, per = gridDim.x / 8;
+  const int64_t t = (b % 8) * per + b / 8;
+  return t < total ? t : -1;
 }
 
 constexpr int TK = 8;   // real lines (x slots) per y-FFT block
 constexpr int TM = 4;   // complex lines (y wavenumbers) per z-FFT block
 
 // ---------------------------------------------------------------- y forward
-// block: (slot tile of TK consecutive interleaved positions, z index i)
+// tile: (TK consecutive interleaved positions, z index i); wave w transforms
+// lines 2w and 2w+1 of the tile
 __global__ void __launch_bounds__(256) k_yfft_forward(PoissonArgs a) {
   extern __shared__ double2 lds[];
   const Geom g = a.g;
   const RealPlan &py = a.py;
   const int H = py.H, N = py.N;
-  const int s0 = blockIdx.x * TK, i = blockIdx.y;
+  const int ntile = g.ldx / TK;
+  const int64_t tile = fft_tile((int64_t)ntile * g.nzm);
+  if (tile < 0) return;
+  const int s0 = (int)(tile % ntile) * TK, i = (int)(tile / ntile);
   const double usdtal = 1.0 / (a.sc.dt * a.sc.al);
   for (int idx = threadIdx.x; idx < TK * N; idx += blockDim.x) {
     const int q = idx % TK, j = idx / TK;
@@ -140,7 +167,8 @@ __global__ void __launch_bounds__(256) k_yfft_forward(PoissonArgs a) {
     }
   }
   __syncthreads();
-  lds_fft(lds, TK, py.c, -1.0);
+  lds_fft_wave(lds + (threadIdx.x / 64) * (TK / 4) * H, TK / 4, py.c, -1.0, threadIdx.x % 64);
+  __syncthreads();
   const int nymh = a.nymh;
   for (int idx = threadIdx.x; idx < TK * nymh; idx += blockDim.x) {
     const int q = idx / nymh, m = idx - q * nymh;
@@ -162,11 +190,14 @@ __global__ void __launch_bounds__(256) k_yfft_forward(PoissonArgs a) {
 }
 
 // ---------------------------------------------------------------- z transforms
-// block: (tile of TM wavenumbers m, k)
+// tile: (TM wavenumbers m, k); wave w transforms line w
 __global__ void __launch_bounds__(256) k_zfft(PoissonArgs a, double sign) {
   extern __shared__ double2 lds[];
   const int N = a.pz.n;
-  const int m0 = blockIdx.x * TM, k = blockIdx.y;
+  const int ntile = (a.nymh + TM - 1) / TM;
+  const int64_t tile = fft_tile((int64_t)ntile * a.g.nxm);
+  if (tile < 0) return;
+  const int m0 = (int)(tile % ntile) * TM, k = (int)(tile / ntile);
   double2 *S = a.S + (int64_t)k * a.g.nzm * a.ldm;
   for (int idx = threadIdx.x; idx < TM * N; idx += blockDim.x) {
     const int q = idx % TM, nn = idx / TM;
@@ -176,7 +207,8 @@ __global__ void __launch_bounds__(256) k_zfft(PoissonArgs a, double sign) {
     lds[q * N + a.pz.rev[nn]] = v;
   }
   __syncthreads();
-  lds_fft(lds, TM, a.pz, sign);
+  lds_fft_wave(lds + (threadIdx.x / 64) * N, 1, a.pz, sign, threadIdx.x % 64);
+  __syncthreads();
   for (int idx = threadIdx.x; idx < TM * N; idx += blockDim.x) {
     const int q = idx % TM, nn = idx / TM;
     const int m = m0 + q;
@@ -239,7 +271,10 @@ __global__ void __launch_bounds__(256) k_yfft_inverse(PoissonArgs a) {
   const Geom g = a.g;
   const RealPlan &py = a.py;
   const int H = py.H, N = py.N;
-  const int s0 = blockIdx.x * TK, i = blockIdx.y;
+  const int ntile = g.ldx / TK;
+  const int64_t tile = fft_tile((int64_t)ntile * g.nzm);
+  if (tile < 0) return;
+  const int s0 = (int)(tile % ntile) * TK, i = (int)(tile / ntile);
   const int nymh = a.nymh;
   for (int idx = threadIdx.x; idx < TK * H; idx += blockDim.x) {
     const int q = idx / H, t = idx - q * H;
@@ -266,7 +301,8 @@ __global__ void __launch_bounds__(256) k_yfft_inverse(PoissonArgs a) {
     lds[q * H + py.c.rev[t]] = z;
   }
   __syncthreads();
-  lds_fft(lds, TK, py.c, +1.0);
+  lds_fft_wave(lds + (threadIdx.x / 64) * (TK / 4) * H, TK / 4, py.c, +1.0, threadIdx.x % 64);
+  __syncthreads();
   for (int idx = threadIdx.x; idx < TK * N; idx += blockDim.x) {
     const int q = idx % TK, j = idx / TK;
     const int s = s0 + q, k = k_of_pos(s, g.L);
@@ -294,14 +330,16 @@ hipError_t fft_configure_lds(size_t bytes) {
   return hipFuncSetAttribute((const void *)k_zfft, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
+static dim3 tile_grid(int64_t total) { return dim3((unsigned)(((total + 7) / 8) * 8)); }
+
 hipError_t launch_yfft_forward(const PoissonArgs &a, hipStream_t st) {
-  dim3 grid(a.g.ldx / TK, a.g.nzm);
+  dim3 grid = tile_grid((int64_t)(a.g.ldx / TK) * a.g.nzm);
   size_t shm = sizeof(double2) * (size_t)TK * a.py.H;
   hipLaunchKernelGGL(k_yfft_forward, grid, dim3(256), shm, st, a);
   return hipGetLastError();
 }
 hipError_t launch_zfft(const PoissonArgs &a, int sign, hipStream_t st) {
-  dim3 grid((a.nymh + TM - 1) / TM, a.g.nxm);
+  dim3 grid = tile_grid((int64_t)((a.nymh + TM - 1) / TM) * a.g.nxm);
   size_t shm = sizeof(double2) * (size_t)TM * a.pz.n;
   hipLaunchKernelGGL(k_zfft, grid, dim3(256), shm, st, a, (double)sign);
   return hipGetLastError();
@@ -312,7 +350,7 @@ hipError_t launch_xsolve_pressure(const PoissonArgs &a, hipStream_t st) {
   return hipGetLastError();
 }
 hipError_t launch_yfft_inverse(const PoissonArgs &a, hipStream_t st) {
-  dim3 grid(a.g.ldx / TK, a.g.nzm);
+  dim3 grid = tile_grid((int64_t)(a.g.ldx / TK) * a.g.nzm);
   size_t shm = sizeof(double2) * (size_t)TK * a.py.H;
   hipLaunchKernelGGL(k_yfft_inverse, grid, dim3(256), shm, st, a);
   return hipGetLastError();

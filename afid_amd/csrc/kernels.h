@@ -46,9 +46,12 @@ struct CorrectArgs {
 
 // One complex FFT length with its radix stages, twiddles exp(-2 pi i t/n)
 // and the mixed-radix digit reversal of the in-place decimation-in-time order.
+// mdiv[s] = ceil(2^32 / m_s) for the span m_s of stage s (0 when m_s = 1):
+// floor(w / m_s) = umulhi(w, mdiv[s]) exactly for w, m_s < 2^16.
 struct FFTPlan {
   int n, nf;
   int f[16];
+  unsigned mdiv[16];
   const double2 *tw;
   const int *rev;
 };

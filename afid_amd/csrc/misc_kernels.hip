@@ -46,59 +46,62 @@ __global__ void k_finalize(const double *partial, double *out, int nq, const int
   (void)ops_dummy;
 }
 
-// iterate over interior points (k < kmax) of all columns, grid-stride
-#define FOR_POINTS(g, kmax)                                                            \
-  for (int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; gid < (g).ncol * (g).ldx; \
-       gid += (int64_t)gridDim.x * blockDim.x)                                          \
-    if (k_of_pos((int)(gid % (g).ldx), (g).L) < (kmax))
+// Interior points (k < kmax) of all columns: each of the RBLK blocks walks a
+// contiguous range of the strip column order (XCD-remapped, so y/z neighbour
+// columns come from the same L2), its threads covering the stored positions.
+template <typename F>
+__device__ __forceinline__ void for_points(const Geom &g, int kmax, F &&f) {
+  const int64_t cpb = (g.ncol + RBLK - 1) / RBLK;
+  const int64_t b = xcd_block(RBLK);
+  if (b < 0) return;
+  const int64_t c1 = min((b + 1) * cpb, g.ncol);
+  for (int64_t c = b * cpb; c < c1; c++) {
+    int j, i;
+    strip_col(c, g, j, i);
+    const int64_t C = col_off(j, i, g);
+    for (int s = threadIdx.x; s < g.ldx; s += blockDim.x) {
+      const int k = k_of_pos(s, g.L);
+      if (k < kmax) f(j, i, C, s, k);
+    }
+  }
+}
 
-__global__ void k_red_cfl(ReduceArgs a) {
+__global__ void __launch_bounds__(256) k_red_cfl(ReduceArgs a) {
   const Geom g = a.g;
   double m = 0.0;
-  FOR_POINTS(g, g.nxm) {
-    const int64_t col = gid / g.ldx;
-    const int s = (int)(gid % g.ldx), k = k_of_pos(s, g.L);
-    const int j = (int)(col % g.nym), i = (int)(col / g.nym);
-    const int64_t C = col * g.ldx;
+  for_points(g, g.nxm, [&](int j, int i, int64_t C, int s, int k) {
     const int64_t CJP = col_off(wrapp(j + 1, g.nym), i, g), CIP = col_off(j, wrapp(i + 1, g.nzm), g);
     const int skp = pos_of_k(k + 1, g.L);
     double qcf = (fabs((a.f.vz[C + s] + a.f.vz[CIP + s]) * 0.5 * a.dz) +
                   fabs((a.f.vy[C + s] + a.f.vy[CJP + s]) * 0.5 * a.dy) +
                   fabs((a.f.vx[C + s] + a.f.vx[C + skp]) * 0.5 * a.kc[KC_UDX3M * g.ldx + s]));
     m = fmax(m, qcf);
-  }
+  });
   m = block_reduce(m, OpMax(), 0);
   if (threadIdx.x == 0) a.partial[blockIdx.x] = m;
 }
 
-__global__ void k_red_div(ReduceArgs a) {
+__global__ void __launch_bounds__(256) k_red_div(ReduceArgs a) {
   const Geom g = a.g;
   double m = 0.0;
-  FOR_POINTS(g, g.nxm) {
-    const int64_t col = gid / g.ldx;
-    const int s = (int)(gid % g.ldx), k = k_of_pos(s, g.L);
-    const int j = (int)(col % g.nym), i = (int)(col / g.nym);
-    const int64_t C = col * g.ldx;
+  for_points(g, g.nxm, [&](int j, int i, int64_t C, int s, int k) {
     const int64_t CJP = col_off(wrapp(j + 1, g.nym), i, g), CIP = col_off(j, wrapp(i + 1, g.nzm), g);
     const int skp = pos_of_k(k + 1, g.L);
     double dq = (a.f.vz[CIP + s] - a.f.vz[C + s]) * a.dz + (a.f.vy[CJP + s] - a.f.vy[C + s]) * a.dy +
                 (a.f.vx[C + skp] - a.f.vx[C + s]) * a.kc[KC_UDX3M * g.ldx + s];
     m = fmax(m, fabs(dq));
-  }
+  });
   m = block_reduce(m, OpMax(), 0);
   if (threadIdx.x == 0) a.partial[blockIdx.x] = m;
 }
 
 // GlobalQuantities.F90:38-56 sums/extrema: q0 anusin, q1 tempm, q2 vx2, q3 vy2,
 // q4 vz2, q5 all2 (sums); q6..q8 vmax(vz,vy,vx), q9 tempmax (max); q10 tempmin (min)
-__global__ void k_red_global(ReduceArgs a) {
+__global__ void __launch_bounds__(256) k_red_global(ReduceArgs a) {
   const Geom g = a.g;
   double an = 0, tm = 0, x2 = 0, y2 = 0, z2 = 0, a2 = 0, mz = 0, my = 0, mx = 0, tmax = -HUGE_VAL,
          tmin = HUGE_VAL;
-  FOR_POINTS(g, g.nxm) {
-    const int64_t col = gid / g.ldx;
-    const int s = (int)(gid % g.ldx), k = k_of_pos(s, g.L);
-    const int64_t C = col * g.ldx;
+  for_points(g, g.nxm, [&](int, int, int64_t C, int s, int k) {
     const int skp = pos_of_k(k + 1, g.L);
     const double fac2 = a.g3rc[k];
     const double vx = a.f.vx[C + s], vy = a.f.vy[C + s], vz = a.f.vz[C + s], T = a.f.temp[C + s];
@@ -115,7 +118,7 @@ __global__ void k_red_global(ReduceArgs a) {
     y2 += fac2 * (vy * vy);
     z2 += fac2 * (vz * vz);
     a2 += fac2 * (vz * vz + vy * vy + vx * vx);
-  }
+  });
   double v[11] = {an, tm, x2, y2, z2, a2, mz, my, mx, tmax, tmin};
   for (int q = 0; q < 11; q++) {
     double r = q < 6 ? block_reduce(v[q], OpSum(), 0)
