@@ -3,8 +3,10 @@ the Fortran host afid_run on top of it.
 
 hipcc compiles each translation unit for gfx950 and links one shared library
 into afid_amd/lib/.  Packed-FP32 instructions are disabled for all device code.
-amdflang compiles afid_amd/fortran/ (iso_c_binding module + main program)
-into afid_amd/lib/afid_run, linked against the library with an $ORIGIN rpath.
+g++ builds afid_amd/io/afid_io.cpp (continuation files, HDF5 C library) into
+afid_amd/lib/libafid_io.so.  amdflang compiles afid_amd/fortran/
+(iso_c_binding module + main program) into afid_amd/lib/afid_run, linked
+against both libraries with an $ORIGIN rpath.
 """
 from __future__ import annotations
 
@@ -54,8 +56,29 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")
     os.replace(tmp, LIB)
+    build_io()
     build_fortran_host()
     return LIB
+
+
+IO_LIB = os.path.join(LIBDIR, "libafid_io.so")
+HDF5_ROOT = os.environ.get("AFID_HDF5_ROOT", "/opt/conda")
+
+
+def build_io() -> str:
+    """libafid_io.so: continuation files over the C ABI and HDF5's C library."""
+    tmp = IO_LIB + ".tmp"
+    # libstdc++ is linked statically: the HDF5 directory on the rpath also holds
+    # an older libstdc++ that must not be the one the HIP runtime gets.
+    cmd = ["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-static-libstdc++", "-static-libgcc",
+           f"-I{HDF5_ROOT}/include",
+           os.path.join(HERE, "io", "afid_io.cpp"), f"-L{LIBDIR}", "-lafid_hip", f"-L{HDF5_ROOT}/lib", "-lhdf5",
+           "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{HDF5_ROOT}/lib", "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed for afid_io.cpp:\n{r.stderr[-4000:]}")
+    os.replace(tmp, IO_LIB)
+    return IO_LIB
 
 
 FORTRAN_SOURCES = ["afid_hip.F90", "afid_main.F90"]
@@ -69,7 +92,7 @@ def build_fortran_host() -> str:
     os.makedirs(moddir, exist_ok=True)
     tmp = RUN + ".tmp"
     cmd = [fc, "-cpp", "-O2", "-J", moddir, *[os.path.join(fdir, f) for f in FORTRAN_SOURCES],
-           f"-L{LIBDIR}", "-lafid_hip", "-Wl,-rpath,$ORIGIN", "-o", tmp]
+           f"-L{LIBDIR}", "-lafid_io", "-lafid_hip", "-Wl,-rpath,$ORIGIN", "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"amdflang failed:\n{r.stderr[-4000:]}")

@@ -568,6 +568,8 @@ const char *afid_strerror(int code) {
     case AFID_ERR_STATE: return "call not valid in this state";
     case AFID_NUM_PIVOT: return "implicit factorisation needs a row interchange";
     case AFID_NUM_SINGULAR: return "singular pressure tridiagonal";
+    case AFID_ERR_IO: return "continuation file could not be read or written";
+    case AFID_ERR_GRID: return "continuation grid differs from this cell (regridding not available)";
     default: return "unknown error";
   }
 }
@@ -657,7 +659,7 @@ int afid_create(afid_ctx **out, const afid_params *p, const afid_metrics *m) {
     if (c->py.H > 1280 || g.nzm > 2560) TRY(AFID_ERR_ARG);
     size_t lds = std::max(sizeof(double2) * 8 * (size_t)c->py.H, sizeof(double2) * 4 * (size_t)g.nzm);
     if (lds > 65536) TRYH(fft_configure_lds(lds));
-    size_t clds = sizeof(double) * 256 * (size_t)g.L;
+    size_t clds = std::max(sizeof(double) * 256 * (size_t)g.L, column_lds_need(g));
     if (clds > 65536) TRYH(column_configure_lds(clds));
   }
   for (int q : {AFID_VX, AFID_VY, AFID_VZ, AFID_TEMP, AFID_PR, AFID_RUX, AFID_RUY, AFID_RUZ, AFID_RUTEMP, AFID_DPH})
@@ -769,7 +771,10 @@ int afid_get_metric(afid_ctx *c, const char *name, double *out, int n) {
   }
   std::map<std::string, double> s = {{"dx", c->dx},   {"dy", c->dy},   {"dz", c->dz},     {"ren", c->ren},
                                      {"pec", c->pec}, {"al", c->al},   {"ga", c->ga},     {"ro", c->ro},
-                                     {"dt", c->dt},   {"beta", c->beta}, {"time", c->time}};
+                                     {"dt", c->dt},   {"beta", c->beta}, {"time", c->time},
+                                     {"ylen", c->p.ylen}, {"zlen", c->p.zlen}, {"alx3", c->p.alx3},
+                                     {"str3", c->p.str3}, {"istr3", (double)c->p.istr3},
+                                     {"ray", c->p.ray}, {"pra", c->p.pra}};
   auto is = s.find(name);
   if (is != s.end() && n >= 1) {
     out[0] = is->second;
@@ -1080,6 +1085,26 @@ int afid_step(afid_ctx *c, int ntime, int idtv, double dtin, double dtmin, doubl
 int afid_get_time(afid_ctx *c, double *t) {
   if (!c || !t) return AFID_ERR_ARG;
   *t = c->time;
+  return AFID_OK;
+}
+
+int afid_set_time(afid_ctx *c, double t) {
+  if (!c) return AFID_ERR_ARG;
+  c->time = t;
+  return AFID_OK;
+}
+
+int afid_get_stale(afid_ctx *c, double *plane) {
+  if (!c || !plane) return AFID_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  HIPCHK(c, hipMemcpy(plane, c->stale, sizeof(double) * c->g.ncol, hipMemcpyDeviceToHost));
+  return AFID_OK;
+}
+
+int afid_set_stale(afid_ctx *c, const double *plane) {
+  if (!c || !plane) return AFID_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  HIPCHK(c, hipMemcpy(c->stale, plane, sizeof(double) * c->g.ncol, hipMemcpyHostToDevice));
   return AFID_OK;
 }
 

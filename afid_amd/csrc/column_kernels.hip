@@ -13,6 +13,7 @@
 // The order of floating-point operations inside each stencil follows the
 // reference expressions (parenthesisation kept), so per-routine results match
 // the reference to rounding.
+#include <algorithm>
 #include <type_traits>
 
 #include "afid_internal.h"
@@ -543,24 +544,47 @@ __global__ void __launch_bounds__(WG)
 // four systems in registers: A q_new = b', with the factor rows staged in LDS.
 // The vz increment at k=1, q_new - q_old, becomes the new stale row and the
 // temperature's rhs(1).
+constexpr int STENCIL_MAX_WG = 512;
+// Coefficient rows staged in LDS by k_stencil_rhs: eight per-k metric rows,
+// then scale, dl, du of the x, y/z and temperature systems.
+enum StencilCoef {
+  SC_UDX3C = 0, SC_UDX3M, SC_AMC, SC_ACC, SC_APC, SC_AMS, SC_ACS, SC_APS,
+  SC_XSC, SC_XDL, SC_XDU, SC_YSC, SC_YDL, SC_YDU, SC_TSC, SC_TDL, SC_TDU, SC_N
+};
+__device__ __forceinline__ int kc_row_of(int r) {
+  const int rows[8] = {KC_UDX3C, KC_UDX3M, KC_AM3CK, KC_AC3CK, KC_AP3CK, KC_AM3SK, KC_AC3SK, KC_AP3SK};
+  return rows[r];
+}
+__device__ __forceinline__ int fac_row_of(int q) { return q == 0 ? FR_SCALE : (q == 1 ? FR_DL : FR_DU); }
+static size_t stencil_lds(const Geom &g) { return sizeof(double) * SC_N * (size_t)g.ldx; }
 template <typename OffT, bool USE_RU>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(STENCIL_MAX_WG)
     k_stencil_rhs(FusedArgs a, const double *__restrict__ vx, const double *__restrict__ vy,
                   const double *__restrict__ vz, const double *__restrict__ T, const double *__restrict__ pr,
                   double *__restrict__ rux, double *__restrict__ ruy, double *__restrict__ ruz,
                   double *__restrict__ rut, double *__restrict__ bx, double *__restrict__ by,
                   double *__restrict__ bz, double *__restrict__ bt, const double *__restrict__ stalep,
                   const double *__restrict__ kc, const double *__restrict__ facp) {
+  extern __shared__ double cf[];   // [SC_N][ldx] per-k coefficient rows
   const Geom g = a.g;
-  const int64_t col = xcd_block(g.ncol);   // one column per block, strip order
-  if (col < 0) return;
-  int jc, ic;
-  strip_col(col, g, jc, ic);
+  const int nstrip = (g.nym + STRIP_W - 1) / STRIP_W;
+  const int64_t row = xcd_block((int64_t)nstrip * g.nzm);   // (strip, z) rows, z fastest
+  if (row < 0) return;
+  const int strip = (int)(row / g.nzm), ic = (int)(row - (int64_t)strip * g.nzm);
   const int L = g.L, ldx = g.ldx, nx = g.nx, nxm = g.nxm;
-  const int jm = wrapm(jc - 1, g.nym), jp = wrapp(jc + 1, g.nym);
+  // the coefficient rows depend on k only: stage them once for the block's columns
+  for (int idx = threadIdx.x; idx < SC_N * ldx; idx += blockDim.x) {
+    const int r = idx / ldx, s = idx - r * ldx;
+    const int src = r < SC_XSC ? kc_row_of(r) * ldx : (((r - SC_XSC) / 3) * FR_N + fac_row_of((r - SC_XSC) % 3)) * ldx;
+    cf[idx] = (r < SC_XSC ? kc : facp)[src + s];
+  }
+  __syncthreads();
   const int im = wrapm(ic - 1, g.nzm), ip = wrapp(ic + 1, g.nzm);
   const OffT cb = (OffT)ldx * sizeof(double);
   auto colb = [&](int jj, int ii) { return ((OffT)ii * (OffT)g.nym + (OffT)jj) * cb; };
+  const int j0 = strip * STRIP_W, j1 = min(j0 + STRIP_W, g.nym);
+  for (int jc = j0; jc < j1; jc++) {
+  const int jm = wrapm(jc - 1, g.nym), jp = wrapp(jc + 1, g.nym);
   const OffT C = colb(jc, ic), CJM = colb(jm, ic), CJP = colb(jp, ic), CIM = colb(jc, im), CIP = colb(jc, ip),
              CJMIP = colb(jm, ip), CJPIM = colb(jp, im);
   const double stale0 = stalep[(int64_t)ic * g.nym + jc];   // stale row of this column
@@ -574,8 +598,7 @@ __global__ void __launch_bounds__(256)
     const bool inx = k >= 1 && k <= nx - 2;   // interior rows of the nx systems
     const bool iny = k <= nxm - 1;            // rows of the nxm systems
     const StepScalars &c = a.sc;
-    const OffT rowb = (OffT)ldx * sizeof(double);
-    auto kcv = [&](int row) { return ldf(kc, (OffT)row * rowb + sb); };
+    auto cfv = [&](int r) { return cf[r * ldx + s]; };
     // centre column, k-1 / k / k+1
     const double vxm = ldf(vx, C + smb), vxc = ldf(vx, C + sb), vxp = ldf(vx, C + spb);
     const double vym = ldf(vy, C + smb), vyc = ldf(vy, C + sb), vyp = ldf(vy, C + spb);
@@ -592,11 +615,9 @@ __global__ void __launch_bounds__(256)
     // neighbour columns at k-1 / k+1
     const double vyjpm = ldf(vy, CJP + smb), vzipm = ldf(vz, CIP + smb);
     const double vxjmp = ldf(vx, CJM + spb), vximp = ldf(vx, CIM + spb);
-    const double udx3c = kcv(KC_UDX3C), udx3m = kcv(KC_UDX3M);
-    const double amc = kcv(KC_AM3CK), acc = kcv(KC_AC3CK), apc = kcv(KC_AP3CK);
-    const double ams = kcv(KC_AM3SK), acs = kcv(KC_AC3SK), aps = kcv(KC_AP3SK);
-    const double *fX = facp + (SYS_X * FR_N) * ldx, *fY = facp + (SYS_YZ * FR_N) * ldx, *fT = facp + (SYS_T * FR_N) * ldx;
-    const OffT fSC = FR_SCALE * rowb + sb, fDL = FR_DL * rowb + sb, fDU = FR_DU * rowb + sb;
+    const double udx3c = cfv(SC_UDX3C), udx3m = cfv(SC_UDX3M);
+    const double amc = cfv(SC_AMC), acc = cfv(SC_ACC), apc = cfv(SC_APC);
+    const double ams = cfv(SC_AMS), acs = cfv(SC_ACS), aps = cfv(SC_APS);
 
     // ---- vx (ExplicitTermsVX.F90:55-101, ImplicitAndUpdateVX.F90:47-60)
     {
@@ -611,7 +632,7 @@ __global__ void __launch_bounds__(256)
       double dxp = (prc - prm) * (c.al * udx3c);
       double r = USE_RU ? c.ro * ldf(rux, C + sb) : 0.0;
       double rhs = (c.ga * q + r + c.alre * dxxvx - dxp) * c.dt;
-      double vin = rhs * ldf(fX, fSC) + ((ldf(fX, fDL) * vxm + vxc) + ldf(fX, fDU) * vxp);
+      double vin = rhs * cfv(SC_XSC) + ((cfv(SC_XDL) * vxm + vxc) + cfv(SC_XDU) * vxp);
       double vbd = (k == 0 ? stale0 : 0.0) + vxc;
       stf(rux, C + sb, inx ? q : 0.0);
       stf(bx, C + sb, inx ? vin : vbd);
@@ -630,8 +651,7 @@ __global__ void __launch_bounds__(256)
       double dyp = (prc - prjm) * c.udy_al;
       double r = USE_RU ? c.ro * ldf(ruy, C + sb) : 0.0;
       double rhs = (c.ga * h + r + c.alre * dxxvy - dyp) * c.dt;
-      double dl = ldf(fY, fDL), du = ldf(fY, fDU);
-      double vin = rhs * ldf(fY, fSC) + ((dl * vym + vyc) + du * vyp);
+      double vin = rhs * cfv(SC_YSC) + ((cfv(SC_YDL) * vym + vyc) + cfv(SC_YDU) * vyp);
       stf(ruy, C + sb, iny ? h : 0.0);
       stf(by, C + sb, vin);   // row nx-1 is outside the nxm system and never read
     }
@@ -649,8 +669,7 @@ __global__ void __launch_bounds__(256)
       double dzp = (prc - prim) * c.dz * c.al;
       double r = USE_RU ? c.ro * ldf(ruz, C + sb) : 0.0;
       double rhs = (c.ga * h + r + c.alre * dxxvz - dzp) * c.dt;
-      double dl = ldf(fY, fDL), du = ldf(fY, fDU);
-      double vin = rhs * ldf(fY, fSC) + ((dl * vzm + vzc) + du * vzp);
+      double vin = rhs * cfv(SC_YSC) + ((cfv(SC_YDL) * vzm + vzc) + cfv(SC_YDU) * vzp);
       stf(ruz, C + sb, iny ? h : 0.0);
       stf(bz, C + sb, vin);
     }
@@ -665,10 +684,11 @@ __global__ void __launch_bounds__(256)
       double dxxt = tp * apc + tc * acc + tm * amc;
       double r = USE_RU ? c.ro * ldf(rut, C + sb) : 0.0;
       double rhs = (c.ga * h + r + c.alpec * dxxt) * c.dt;
-      double vin = rhs * ldf(fT, fSC) + ((ldf(fT, fDL) * tm + tc) + ldf(fT, fDU) * tp);
+      double vin = rhs * cfv(SC_TSC) + ((cfv(SC_TDL) * tm + tc) + cfv(SC_TDU) * tp);
       stf(rut, C + sb, inx ? h : 0.0);
       stf(bt, C + sb, inx ? vin : tc);   // row 1 gets the vz increment added by the solve pass
     }
+  }
   }
 }
 
@@ -949,41 +969,53 @@ __global__ void k_divergence(RoutineArgs a) {
 }
 
 // CorrectVelocity.F90:28-45 and CorrectPressure.F90:32-52 from dph (= dphhalo).
-// which: 1 velocity, 2 pressure, 3 both (the fused path).  One block per
-// column in strip order (XCD-remapped): the y/z neighbour columns of dph were
-// read by nearby blocks of the same XCD.
-__global__ void __launch_bounds__(256) k_correct(CorrectArgs a, int which) {
+// which: 1 velocity, 2 pressure, 3 both (the fused path).  A block walks the
+// STRIP_W columns of one (strip, z) row (XCD-remapped, z fastest), so the y/z
+// neighbour columns of dph come from the L2 of the same XCD; the four per-k
+// coefficient rows are staged in LDS once per block.
+constexpr int CORRECT_MAX_WG = 512;
+__global__ void __launch_bounds__(CORRECT_MAX_WG) k_correct(CorrectArgs a, int which) {
+  extern __shared__ double cr[];   // udx3c, amphk, acphk, apphk rows
   const Geom g = a.g;
-  const int64_t col = xcd_block(g.ncol);
-  if (col < 0) return;
-  int j, i;
-  strip_col(col, g, j, i);
-  const int jm = wrapm(j - 1, g.nym), jp = wrapp(j + 1, g.nym);
-  const int im = wrapm(i - 1, g.nzm), ip = wrapp(i + 1, g.nzm);
-  const int64_t C = col_off(j, i, g), CJM = col_off(jm, i, g), CJP = col_off(jp, i, g), CIM = col_off(j, im, g),
-                CIP = col_off(j, ip, g);
+  const int nstrip = (g.nym + STRIP_W - 1) / STRIP_W;
+  const int64_t row = xcd_block((int64_t)nstrip * g.nzm);
+  if (row < 0) return;
+  const int strip = (int)(row / g.nzm), i = (int)(row - (int64_t)strip * g.nzm);
   const int ldx = g.ldx;
+  for (int idx = threadIdx.x; idx < 4 * ldx; idx += blockDim.x) {
+    const int r = idx / ldx, s = idx - r * ldx;
+    const int src = r == 0 ? KC_UDX3C : (r == 1 ? KC_AMPHK : (r == 2 ? KC_ACPHK : KC_APPHK));
+    cr[idx] = a.kc[src * ldx + s];
+  }
+  __syncthreads();
+  const int im = wrapm(i - 1, g.nzm), ip = wrapp(i + 1, g.nzm);
   const double *__restrict__ dph = a.dph;
   const StepScalars &c = a.sc;
-  for (int s = threadIdx.x; s < ldx; s += blockDim.x) {
-    const int k = k_of_pos(s, g.L);
-    if (k > g.nxm - 1) continue;
-    const int skm = k == 0 ? s : pos_of_k(k - 1, g.L);
-    const int skp = k == g.nxm - 1 ? s : pos_of_k(k + 1, g.L);
-    const double d = dph[C + s], dkm = dph[C + skm], djm = dph[CJM + s], dim_ = dph[CIM + s];
-    if (which & 1) {
-      double usukm = c.al * c.dt * a.kc[KC_UDX3C * ldx + s];
-      double udy = c.al * c.dt * c.dy, udz = c.al * c.dt * c.dz;
-      a.vx[C + s] = a.vx[C + s] - (d - dkm) * usukm;
-      a.vy[C + s] = a.vy[C + s] - (d - djm) * udy;
-      a.vz[C + s] = a.vz[C + s] - (d - dim_) * udz;
-    }
-    if (which & 2) {
-      double be = c.al * c.beta;
-      double amm = a.kc[KC_AMPHK * ldx + s], acc = a.kc[KC_ACPHK * ldx + s], app = a.kc[KC_APPHK * ldx + s];
-      a.pr[C + s] = a.pr[C + s] + d -
-                    be * ((dph[CIP + s] - 2.0 * d + dim_) * c.dzq + (dph[CJP + s] - 2.0 * d + djm) * c.dyq +
-                          (dph[C + skp] * app + d * acc + dkm * amm));
+  const int j0 = strip * STRIP_W, j1 = min(j0 + STRIP_W, g.nym);
+  for (int j = j0; j < j1; j++) {
+    const int jm = wrapm(j - 1, g.nym), jp = wrapp(j + 1, g.nym);
+    const int64_t C = col_off(j, i, g), CJM = col_off(jm, i, g), CJP = col_off(jp, i, g), CIM = col_off(j, im, g),
+                  CIP = col_off(j, ip, g);
+    for (int s = threadIdx.x; s < ldx; s += blockDim.x) {
+      const int k = k_of_pos(s, g.L);
+      if (k > g.nxm - 1) continue;
+      const int skm = k == 0 ? s : pos_of_k(k - 1, g.L);
+      const int skp = k == g.nxm - 1 ? s : pos_of_k(k + 1, g.L);
+      const double d = dph[C + s], dkm = dph[C + skm], djm = dph[CJM + s], dim_ = dph[CIM + s];
+      if (which & 1) {
+        double usukm = c.al * c.dt * cr[s];
+        double udy = c.al * c.dt * c.dy, udz = c.al * c.dt * c.dz;
+        a.vx[C + s] = a.vx[C + s] - (d - dkm) * usukm;
+        a.vy[C + s] = a.vy[C + s] - (d - djm) * udy;
+        a.vz[C + s] = a.vz[C + s] - (d - dim_) * udz;
+      }
+      if (which & 2) {
+        double be = c.al * c.beta;
+        double amm = cr[ldx + s], acc = cr[2 * ldx + s], app = cr[3 * ldx + s];
+        a.pr[C + s] = a.pr[C + s] + d -
+                      be * ((dph[CIP + s] - 2.0 * d + dim_) * c.dzq + (dph[CJP + s] - 2.0 * d + djm) * c.dyq +
+                            (dph[C + skp] * app + d * acc + dkm * amm));
+      }
     }
   }
 }
@@ -1022,6 +1054,12 @@ hipError_t column_configure_lds(size_t bytes) {
   if (e != hipSuccess) return e;
   AFID_LM_LIST(SETB)
 #undef SETB
+#define SETC(O, R)                                                                                       \
+  e = hipFuncSetAttribute((const void *)k_stencil_rhs<O, R>, hipFuncAttributeMaxDynamicSharedMemorySize,  \
+                          (int)bytes);                                                                   \
+  if (e != hipSuccess) return e;
+  SETC(uint32_t, true) SETC(uint32_t, false) SETC(uint64_t, true) SETC(uint64_t, false)
+#undef SETC
   return hipSuccess;
 }
 
@@ -1059,12 +1097,17 @@ hipError_t launch_fused_update(const FusedArgs &a, hipStream_t st) {
 
 hipError_t launch_stencil_rhs(const FusedArgs &a, hipStream_t st) {
   const int64_t n = a.g.ncol * a.g.ldx;
-  const dim3 grid((unsigned)(((a.g.ncol + 7) / 8) * 8));
+  const int64_t nrow = (int64_t)((a.g.nym + STRIP_W - 1) / STRIP_W) * a.g.nzm;
+  const dim3 grid((unsigned)(((nrow + 7) / 8) * 8));
+  // a block walks the STRIP_W columns of one (strip, z) row; one wave per 64
+  // stored points of a column, up to STENCIL_MAX_WG
+  const dim3 block((unsigned)std::min(STENCIL_MAX_WG, (a.g.ldx + 63) / 64 * 64));
+  const size_t shm = stencil_lds(a.g);
   const bool small = (uint64_t)n * sizeof(double) < (1ull << 32);
   const bool ru = a.sc.use_ru != 0;
 #define SR(O, R)                                                                                         \
   if (std::is_same<O, uint32_t>::value == small && R == ru) {                                          \
-    hipLaunchKernelGGL((k_stencil_rhs<O, R>), grid, dim3(256), 0, st, a, a.f.vx, a.f.vy, a.f.vz, a.f.temp, \
+    hipLaunchKernelGGL((k_stencil_rhs<O, R>), grid, block, shm, st, a, a.f.vx, a.f.vy, a.f.vz, a.f.temp,   \
                        a.f.pr, a.rux, a.ruy, a.ruz, a.rut, a.bx, a.by, a.bz, a.bt, a.stale, a.kc, a.fac);   \
     return hipGetLastError();                                                                          \
   }
@@ -1118,11 +1161,16 @@ hipError_t launch_divergence(const RoutineArgs &a, hipStream_t st) {
   return hipGetLastError();
 }
 hipError_t launch_correct(const CorrectArgs &a, int which, hipStream_t st) {
-  hipLaunchKernelGGL(k_correct, dim3((unsigned)(((a.g.ncol + 7) / 8) * 8)), dim3(256), 0, st, a, which);
+  const int64_t nrow = (int64_t)((a.g.nym + STRIP_W - 1) / STRIP_W) * a.g.nzm;
+  const dim3 block((unsigned)std::min(CORRECT_MAX_WG, (a.g.ldx + 63) / 64 * 64));
+  hipLaunchKernelGGL(k_correct, dim3((unsigned)(((nrow + 7) / 8) * 8)), block, sizeof(double) * 4 * a.g.ldx, st, a,
+                     which);
   return hipGetLastError();
 }
 
-// LDS holds WG*L doubles per block: L <= 80 keeps it within gfx950's 160 KiB.
-int max_supported_L() { return 80; }
+// LDS per block: WG*L doubles (fused update), SC_N*16*L (stencil pass),
+// SYS_N*RW_N*16*L (column solve); L <= 72 keeps all within gfx950's 160 KiB.
+int max_supported_L() { return 72; }
+size_t column_lds_need(const Geom &g) { return std::max(stencil_lds(g), solve_lds(g)); }
 
 }  // namespace afid

@@ -30,6 +30,12 @@ __device__ __forceinline__ double2 twid(const double2 *tw, int idx, double sign)
   return t;
 }
 
+// LDS element order within a line: for lines whose length is a multiple of 8,
+// element e lives at e ^ ((e >> 3) & 7).  A double2 spans 4 of the 32 LDS
+// banks, so 8 consecutive elements fill one bank row; the XOR spreads the
+// strided butterfly accesses of the early radix stages over the rows.
+__device__ __forceinline__ int swz(int e, bool on) { return on ? e ^ ((e >> 3) & 7) : e; }
+
 // LDS writes of this wave are complete and visible to its own later reads.
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -46,6 +52,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 __device__ __forceinline__ void lds_fft_wave(double2 *lines, int nlines, const FFTPlan &pl, double sign,
                                              int lane) {
   const int n = pl.n;
+  const bool sw = (n & 7) == 0;
   const double2 *tw = pl.tw;
   int m = 1;
   for (int st = 0; st < pl.nf; st++) {
@@ -59,56 +66,60 @@ __device__ __forceinline__ void lds_fft_wave(double2 *lines, int nlines, const F
       for (int w = lane; w < nb; w += 64) {
         const int bb = m == 1 ? w : (int)__umulhi((unsigned)w, mdiv);
         const int u = w - bb * m;
-        double2 *base = line + bb * M + u;
+        const int e0 = bb * M + u;
+        double2 *base = line;
         const int t1i = u * tws;
+        // element v of the butterfly: base[X(v)], X(v) = swz(e0 + v*m)
+#define X(v) swz(e0 + (v) * m, sw)
         if (r == 2) {
-          double2 x0 = base[0], x1 = cmul(base[m], twid(tw, t1i, sign));
-          base[0] = cadd(x0, x1);
-          base[m] = csub(x0, x1);
+          double2 x0 = base[X(0)], x1 = cmul(base[X(1)], twid(tw, t1i, sign));
+          base[X(0)] = cadd(x0, x1);
+          base[X(1)] = csub(x0, x1);
         } else if (r == 4) {
-          double2 x0 = base[0];
-          double2 x1 = cmul(base[m], twid(tw, t1i, sign));
-          double2 x2 = cmul(base[2 * m], twid(tw, 2 * t1i, sign));
-          double2 x3 = cmul(base[3 * m], twid(tw, 3 * t1i, sign));
+          double2 x0 = base[X(0)];
+          double2 x1 = cmul(base[X(1)], twid(tw, t1i, sign));
+          double2 x2 = cmul(base[X(2)], twid(tw, 2 * t1i, sign));
+          double2 x3 = cmul(base[X(3)], twid(tw, 3 * t1i, sign));
           double2 a0 = cadd(x0, x2), a1 = csub(x0, x2);
           double2 b0 = cadd(x1, x3), b1 = mul_si(csub(x1, x3), sign);
-          base[0] = cadd(a0, b0);
-          base[m] = cadd(a1, b1);
-          base[2 * m] = csub(a0, b0);
-          base[3 * m] = csub(a1, b1);
+          base[X(0)] = cadd(a0, b0);
+          base[X(1)] = cadd(a1, b1);
+          base[X(2)] = csub(a0, b0);
+          base[X(3)] = csub(a1, b1);
         } else if (r == 3) {
           const double s3 = 0.86602540378443864676372317075294;  // sin(2 pi/3)
-          double2 x0 = base[0];
-          double2 x1 = cmul(base[m], twid(tw, t1i, sign));
-          double2 x2 = cmul(base[2 * m], twid(tw, 2 * t1i, sign));
+          double2 x0 = base[X(0)];
+          double2 x1 = cmul(base[X(1)], twid(tw, t1i, sign));
+          double2 x2 = cmul(base[X(2)], twid(tw, 2 * t1i, sign));
           double2 t1 = cadd(x1, x2);
           double2 t2 = mul_si(csub(x1, x2), sign * s3);
           double2 a = make_double2(x0.x - 0.5 * t1.x, x0.y - 0.5 * t1.y);
-          base[0] = cadd(x0, t1);
-          base[m] = cadd(a, t2);
-          base[2 * m] = csub(a, t2);
+          base[X(0)] = cadd(x0, t1);
+          base[X(1)] = cadd(a, t2);
+          base[X(2)] = csub(a, t2);
         } else {  // r == 5
           const double c1 = 0.30901699437494742410229341718282;   // cos(2 pi/5)
           const double c2 = -0.80901699437494742410229341718282;  // cos(4 pi/5)
           const double s1 = 0.95105651629515357211643933337938;   // sin(2 pi/5)
           const double s2 = 0.58778525229247312916870595463907;   // sin(4 pi/5)
-          double2 x0 = base[0];
-          double2 x1 = cmul(base[m], twid(tw, t1i, sign));
-          double2 x2 = cmul(base[2 * m], twid(tw, 2 * t1i, sign));
-          double2 x3 = cmul(base[3 * m], twid(tw, 3 * t1i, sign));
-          double2 x4 = cmul(base[4 * m], twid(tw, 4 * t1i, sign));
+          double2 x0 = base[X(0)];
+          double2 x1 = cmul(base[X(1)], twid(tw, t1i, sign));
+          double2 x2 = cmul(base[X(2)], twid(tw, 2 * t1i, sign));
+          double2 x3 = cmul(base[X(3)], twid(tw, 3 * t1i, sign));
+          double2 x4 = cmul(base[X(4)], twid(tw, 4 * t1i, sign));
           double2 t1 = cadd(x1, x4), t2 = cadd(x2, x3);
           double2 t3 = csub(x1, x4), t4 = csub(x2, x3);
           double2 a1 = make_double2(x0.x + c1 * t1.x + c2 * t2.x, x0.y + c1 * t1.y + c2 * t2.y);
           double2 a2 = make_double2(x0.x + c2 * t1.x + c1 * t2.x, x0.y + c2 * t1.y + c1 * t2.y);
           double2 b1 = mul_si(make_double2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y), sign);
           double2 b2 = mul_si(make_double2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y), sign);
-          base[0] = cadd(x0, cadd(t1, t2));
-          base[m] = cadd(a1, b1);
-          base[2 * m] = cadd(a2, b2);
-          base[3 * m] = csub(a2, b2);
-          base[4 * m] = csub(a1, b1);
+          base[X(0)] = cadd(x0, cadd(t1, t2));
+          base[X(1)] = cadd(a1, b1);
+          base[X(2)] = cadd(a2, b2);
+          base[X(3)] = csub(a2, b2);
+          base[X(4)] = csub(a1, b1);
         }
+#undef X
       }
     }
     wave_lds_sync();
@@ -141,6 +152,7 @@ __global__ void __launch_bounds__(256) k_yfft_forward(PoissonArgs a) {
   const int64_t tile = fft_tile((int64_t)ntile * g.nzm);
   if (tile < 0) return;
   const int s0 = (int)(tile % ntile) * TK, i = (int)(tile / ntile);
+  const bool hsw = (H & 7) == 0;
   const double usdtal = 1.0 / (a.sc.dt * a.sc.al);
   for (int idx = threadIdx.x; idx < TK * N; idx += blockDim.x) {
     const int q = idx % TK, j = idx / TK;
@@ -159,11 +171,11 @@ __global__ void __launch_bounds__(256) k_yfft_forward(PoissonArgs a) {
       }
     }
     if (py.half) {
-      double2 *e = &lds[q * H + py.c.rev[j >> 1]];
+      double2 *e = &lds[q * H + swz(py.c.rev[j >> 1], hsw)];
       if (j & 1) e->y = v;
       else e->x = v;
     } else {
-      lds[q * H + py.c.rev[j]] = make_double2(v, 0.0);
+      lds[q * H + swz(py.c.rev[j], hsw)] = make_double2(v, 0.0);
     }
   }
   __syncthreads();
@@ -176,14 +188,14 @@ __global__ void __launch_bounds__(256) k_yfft_forward(PoissonArgs a) {
     if (k >= g.nxm) continue;
     double2 X;
     if (py.half) {
-      double2 zm = lds[q * H + (m == H ? 0 : m)];
-      double2 zc = conj2(lds[q * H + (m == 0 ? 0 : H - m)]);
+      double2 zm = lds[q * H + swz(m == H ? 0 : m, hsw)];
+      double2 zc = conj2(lds[q * H + swz(m == 0 ? 0 : H - m, hsw)]);
       double2 E = make_double2(0.5 * (zm.x + zc.x), 0.5 * (zm.y + zc.y));
       double2 D = csub(zm, zc);
       double2 O = make_double2(0.5 * D.y, -0.5 * D.x);  // D / (2i)
       X = cadd(E, cmul(py.post[m], O));
     } else {
-      X = lds[q * H + m];
+      X = lds[q * H + swz(m, hsw)];
     }
     a.S[((int64_t)k * g.nzm + i) * a.ldm + m] = X;
   }
@@ -198,13 +210,14 @@ __global__ void __launch_bounds__(256) k_zfft(PoissonArgs a, double sign) {
   const int64_t tile = fft_tile((int64_t)ntile * a.g.nxm);
   if (tile < 0) return;
   const int m0 = (int)(tile % ntile) * TM, k = (int)(tile / ntile);
+  const bool zsw = (N & 7) == 0;
   double2 *S = a.S + (int64_t)k * a.g.nzm * a.ldm;
   for (int idx = threadIdx.x; idx < TM * N; idx += blockDim.x) {
     const int q = idx % TM, nn = idx / TM;
     const int m = m0 + q;
     double2 v = make_double2(0.0, 0.0);
     if (m < a.nymh) v = S[(int64_t)nn * a.ldm + m];
-    lds[q * N + a.pz.rev[nn]] = v;
+    lds[q * N + swz(a.pz.rev[nn], zsw)] = v;
   }
   __syncthreads();
   lds_fft_wave(lds + (threadIdx.x / 64) * N, 1, a.pz, sign, threadIdx.x % 64);
@@ -212,7 +225,7 @@ __global__ void __launch_bounds__(256) k_zfft(PoissonArgs a, double sign) {
   for (int idx = threadIdx.x; idx < TM * N; idx += blockDim.x) {
     const int q = idx % TM, nn = idx / TM;
     const int m = m0 + q;
-    if (m < a.nymh) S[(int64_t)nn * a.ldm + m] = lds[q * N + nn];
+    if (m < a.nymh) S[(int64_t)nn * a.ldm + m] = lds[q * N + swz(nn, zsw)];
   }
 }
 
@@ -275,6 +288,7 @@ __global__ void __launch_bounds__(256) k_yfft_inverse(PoissonArgs a) {
   const int64_t tile = fft_tile((int64_t)ntile * g.nzm);
   if (tile < 0) return;
   const int s0 = (int)(tile % ntile) * TK, i = (int)(tile / ntile);
+  const bool hsw = (H & 7) == 0;
   const int nymh = a.nymh;
   for (int idx = threadIdx.x; idx < TK * H; idx += blockDim.x) {
     const int q = idx / H, t = idx - q * H;
@@ -298,7 +312,7 @@ __global__ void __launch_bounds__(256) k_yfft_inverse(PoissonArgs a) {
         }
       }
     }
-    lds[q * H + py.c.rev[t]] = z;
+    lds[q * H + swz(py.c.rev[t], hsw)] = z;
   }
   __syncthreads();
   lds_fft_wave(lds + (threadIdx.x / 64) * (TK / 4) * H, TK / 4, py.c, +1.0, threadIdx.x % 64);
@@ -309,10 +323,10 @@ __global__ void __launch_bounds__(256) k_yfft_inverse(PoissonArgs a) {
     if (k >= g.nxm) continue;
     double v;
     if (py.half) {
-      double2 z = lds[q * H + (j >> 1)];
+      double2 z = lds[q * H + swz(j >> 1, hsw)];
       v = (j & 1) ? z.y : z.x;
     } else {
-      v = lds[q * H + j].x;
+      v = lds[q * H + swz(j, hsw)].x;
     }
     a.dph[col_off(j, i, g) + s] = v;
   }

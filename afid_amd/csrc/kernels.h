@@ -90,6 +90,7 @@ hipError_t launch_rhs(const RoutineArgs &a, int which, hipStream_t st);
 hipError_t launch_divergence(const RoutineArgs &a, hipStream_t st);
 hipError_t launch_correct(const CorrectArgs &a, int which, hipStream_t st);
 int max_supported_L();
+size_t column_lds_need(const Geom &g);   // largest dynamic LDS of the split-update kernels
 hipError_t column_configure_lds(size_t bytes);
 
 hipError_t launch_yfft_forward(const PoissonArgs &a, hipStream_t st);

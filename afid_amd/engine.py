@@ -24,6 +24,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libafid_hip.so")
+IO_LIB_PATH = os.path.join(HERE, "lib", "libafid_io.so")
 
 AFID_VX, AFID_VY, AFID_VZ, AFID_TEMP, AFID_PR = range(5)
 AFID_RUX, AFID_RUY, AFID_RUZ, AFID_RUTEMP, AFID_DPH, AFID_QCAP, AFID_DQ, AFID_HRO, AFID_RHS = range(5, 14)
@@ -73,6 +74,9 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "afid_time_marcher": ([vp, d], i),
         "afid_step": ([vp, i, i, d, d, d, d, pd, ctypes.POINTER(i)], i),
         "afid_get_time": ([vp, pd], i),
+        "afid_set_time": ([vp, d], i),
+        "afid_get_stale": ([vp, pd], i),
+        "afid_set_stale": ([vp, pd], i),
         "afid_calc_max_cfl": ([vp, pd], i),
         "afid_check_divergence": ([vp, pd], i),
         "afid_calc_plate_nu": ([vp, pd, pd], i),
@@ -98,12 +102,54 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     return L
 
 
-def exported_symbols() -> list[str]:
-    """Entry points declared in include/afid.h (for the export check)."""
-    hdr = os.path.join(os.path.dirname(HERE), "include", "afid.h")
+_io = None
+
+
+def load_io_library(path: str = IO_LIB_PATH) -> ctypes.CDLL:
+    """Load libafid_io.so (continuation files); raises AfidError if not built."""
+    global _io
+    if _io is not None:
+        return _io
+    load_library()
+    if not os.path.exists(path):
+        raise AfidError(f"{path} not found: build it with afid_amd.build.build()")
+    L = ctypes.CDLL(path)
+    vp, i, pd, cp = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_char_p
+    sig = {"afid_write_continuation": ([vp, cp], i), "afid_read_continuation": ([vp, cp], i),
+           "afid_h5_write_field": ([cp, cp, pd, i, i, i], i), "afid_h5_read_field": ([cp, cp, pd, i, i, i], i)}
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _io = L
+    return L
+
+
+def exported_symbols(header: str = "afid.h") -> list[str]:
+    """Entry points declared in include/<header> (for the export check)."""
+    hdr = os.path.join(os.path.dirname(HERE), "include", header)
     import re
     names = re.findall(r"^(?:int|const char \*)\s*(afid_\w+)\s*\(", open(hdr).read(), flags=re.M)
     return names
+
+
+def h5_write_field(path: str, dset: str, a: np.ndarray) -> None:
+    """One FP64 dataset of Fortran shape (n1, n2, n3) from a numpy [i, j, k] array."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    n3, n2, n1 = a.shape
+    rc = load_io_library().afid_h5_write_field(path.encode(), dset.encode(), _dp(a), n1, n2, n3)
+    if rc:
+        raise AfidError(f"h5_write_field: {load_library().afid_strerror(rc).decode()} ({rc})")
+
+
+def h5_read_field(path: str, dset: str, shape: tuple) -> np.ndarray:
+    """Reads a dataset of Fortran shape (n1, n2, n3) into a numpy [i, j, k] array of `shape`."""
+    a = np.zeros(shape)
+    n3, n2, n1 = shape
+    rc = load_io_library().afid_h5_read_field(path.encode(), dset.encode(), _dp(a), n1, n2, n3)
+    if rc:
+        raise AfidError(f"h5_read_field: {load_library().afid_strerror(rc).decode()} ({rc})")
+    return a
 
 
 @dataclass
@@ -256,6 +302,17 @@ class Cell:
         t = ctypes.c_double()
         self._check(self.L.afid_get_time(self.h, ctypes.byref(t)), "get_time")
         return t.value
+
+    @time.setter
+    def time(self, t: float):
+        self._check(self.L.afid_set_time(self.h, t), "set_time")
+
+    # ---- continuation files (WriteFlowField.F90 / ReadFlowField.F90) -------------
+    def WriteFlowField(self, directory: str = "."):
+        self._check(load_io_library().afid_write_continuation(self.h, directory.encode()), "WriteFlowField")
+
+    def ReadFlowField(self, directory: str = "."):
+        self._check(load_io_library().afid_read_continuation(self.h, directory.encode()), "ReadFlowField")
 
     # ---- diagnostics -------------------------------------------------------------
     def CalcMaxCFL(self) -> float:

@@ -18,6 +18,7 @@
       integer(c_int), parameter :: AFID_ERR_NOMEM = -3
       integer(c_int), parameter :: AFID_ERR_FFTSIZE = -4
       integer(c_int), parameter :: AFID_ERR_STATE = -5
+      integer(c_int), parameter :: AFID_ERR_IO = -6, AFID_ERR_GRID = -7
       integer(c_int), parameter :: AFID_NUM_PIVOT = 1
       integer(c_int), parameter :: AFID_NUM_SINGULAR = 2
 
@@ -192,6 +193,31 @@
           real(c_double), value :: dt, dtmin, dtmax, limitcfl
           real(c_double), intent(out) :: dt_used
           integer(c_int), intent(out) :: errorcode
+        end function
+        integer(c_int) function afid_get_time(ctx, time)                &
+     &      bind(C, name='afid_get_time')
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: ctx
+          real(c_double), intent(out) :: time
+        end function
+        integer(c_int) function afid_set_time(ctx, time)                &
+     &      bind(C, name='afid_set_time')
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: ctx
+          real(c_double), value :: time
+        end function
+!       continuation files (include/afid_io.h, libafid_io)
+        integer(c_int) function afid_write_continuation(ctx, dir)       &
+     &      bind(C, name='afid_write_continuation')
+          import :: c_int, c_ptr, c_char
+          type(c_ptr), value :: ctx
+          character(kind=c_char), intent(in) :: dir(*)
+        end function
+        integer(c_int) function afid_read_continuation(ctx, dir)        &
+     &      bind(C, name='afid_read_continuation')
+          import :: c_int, c_ptr, c_char
+          type(c_ptr), value :: ctx
+          character(kind=c_char), intent(in) :: dir(*)
         end function
         integer(c_int) function afid_calc_max_cfl(ctx, cflm)            &
      &      bind(C, name='afid_calc_max_cfl')

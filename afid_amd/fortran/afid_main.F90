@@ -7,7 +7,8 @@
 !  main.F90:153-279 (Factorize check, CFL/dt selection, output cadence,
 !  error codes), TimeMarcher.F90:13-78 (substep sequence), the log files
 !  of CalcPlateNu.F90:49-53, GlobalQuantities.F90:81-97 and
-!  CalcDissipationNu.F90:114-116.
+!  CalcDissipationNu.F90:114-116; ReadFlowField.F90:29-45,160-176 (nread)
+!  and QuitRoutine.F90:27-31 (continuation written at a normal exit).
 !
 !  Usage: afid_run [bou.in] [--seed N] [--routines] [--out DIR]
 !                  [--parse-only] [--abi-check]
@@ -234,11 +235,6 @@
         stop
       end if
 
-      if (nread /= 0) then
-        write(6,*) 'afid_run: restart from continuation files is not ',  &
-     &    'supported by this host (nread must be 0)'
-        error stop 2
-      end if
       errorcode = 0
       if (max(LargestFactor(nym), LargestFactor(nzm)) > 5) then
         write(6,*) 'FFT size in ny or nz is not efficient'
@@ -261,9 +257,19 @@
       p%tempbp = 1.d0
       p%temptp = 0.d0
       call afid_check(afid_create(ctx, p, c_null_ptr), 'afid_create')
-      call afid_check(afid_init_linear_noise(ctx, seed, 1.d-3),           &
-     &  'CreateInitialConditions')
-      time = 0.d0
+      if (nread /= 0) then
+        write(6,*) 'Reading initial condition from file'
+        call afid_check(afid_read_continuation(ctx, '.'//c_null_char),   &
+     &    'ReadFlowField')
+        call afid_check(afid_get_time(ctx, time), 'ReadFlowField')
+        if (ireset /= 0) time = 0.d0
+        tmax = tmax + time
+      else
+        write(6,*) 'Creating initial condition'
+        call afid_check(afid_init_linear_noise(ctx, seed, 1.d-3),         &
+     &    'CreateInitialConditions')
+        time = 0.d0
+      end if
       call afid_check(afid_check_divergence(ctx, dmax), 'CheckDivergence')
       write(6,*) ' Initial maximum divergence: ', dmax
 
@@ -331,6 +337,12 @@
         write(6,*) 'Maximum number of timesteps reached'
       end select
       write(6,'(a,i0,a,es23.16)') 'errorcode ', errorcode, ' time ', time
+      if (errorcode == 333 .or. errorcode == 334 .or. errorcode == 555)   &
+     &  then
+        call afid_check(afid_set_time(ctx, time), 'WriteFlowField')
+        call afid_check(afid_write_continuation(ctx, '.'//c_null_char),  &
+     &    'WriteFlowField')
+      end if
       if (len_trim(outdir) > 0) call WriteFields(trim(outdir))
       call afid_check(afid_destroy(ctx), 'afid_destroy')
       end program afid_run

@@ -28,6 +28,9 @@ extern "C" {
 #define AFID_ERR_NOMEM (-3)      /* device allocation failed                 */
 #define AFID_ERR_FFTSIZE (-4)    /* lateral size has a prime factor > 5 (errorcode 444) */
 #define AFID_ERR_STATE (-5)      /* call not valid in the context's state    */
+#define AFID_ERR_IO (-6)         /* continuation file missing or unreadable (afid_io.h) */
+#define AFID_ERR_GRID (-7)       /* continuation written on another grid; the reference
+                                    would interpolate (ReadFlowField.F90:56-141) */
 #define AFID_NUM_PIVOT 1         /* an implicit factorisation would need a row
                                     interchange (dgttrf pivoting) */
 #define AFID_NUM_SINGULAR 2      /* singular pressure tridiagonal (zgttrf info>0),
@@ -76,7 +79,9 @@ const char *afid_strerror(int code);
 int afid_last_hip_error(afid_ctx *ctx);
 /* Sizes: out[0..]= nx, nxm, nym, nzm, device x pitch, bytes of device memory held. */
 int afid_get_dims(afid_ctx *ctx, int64_t *out6);
-/* 1-D metric copies (1-based name as in param.F90), n values into out. */
+/* 1-D metric copies (1-based name as in param.F90), n values into out; the
+ * scalars dx, dy, dz, ren, pec, al, ga, ro, dt, beta, time and the bou.in
+ * values ylen, zlen, alx3, str3, istr3, ray, pra come back with n = 1. */
 int afid_get_metric(afid_ctx *ctx, const char *name, double *out, int n);
 
 /* ---- host <-> device (replaces the module arrays as the data exchange) ----
@@ -130,6 +135,12 @@ int afid_time_marcher(afid_ctx *ctx, double dt);
 int afid_step(afid_ctx *ctx, int ntime, int idtv, double dt, double dtmin, double dtmax,
               double limitcfl, double *dt_used, int *errorcode);
 int afid_get_time(afid_ctx *ctx, double *time);
+int afid_set_time(afid_ctx *ctx, double time);      /* time of a continuation (ReadFlowField.F90:46) */
+/* The stale row rhs(1,j,i) the x-solves read (SolveImpEqnUpdate_X.F90:44):
+ * plane[j + nym*i], j < nym, i < nzm.  Part of the state a restart needs to
+ * continue bit for bit. */
+int afid_get_stale(afid_ctx *ctx, double *plane);
+int afid_set_stale(afid_ctx *ctx, const double *plane);
 
 /* ---- diagnostics (FP64 device reductions) ----------------------------------- */
 int afid_calc_max_cfl(afid_ctx *ctx, double *cflm);          /* CalcMaxCFL.F90:11-48 */

@@ -1,0 +1,39 @@
+/*
+ * afid_io.h -- continuation files of a cell (libafid_io.so, HDF5).
+ *
+ * The reference writes one file per prognostic field, continua_<name>.h5 with
+ * a dataset 'var' of Fortran shape (nx, nym, nzm), and continua_master.h5
+ * with the scalars nx, ny, nz, ylen, zlen, time, istr3, str3
+ * (WriteFlowField.F90:17-45, HdfRoutines.F90:388-477).  These entry points
+ * write and read the same layout.  They also write continua_state.h5: pr,
+ * the histories rux..rutemp and the stale row rhs(1,:,:).  The reference
+ * restarts without them, from pr = 0 and zeroed histories
+ * (ReadFlowField.F90:160-170).  With them a restart continues bit for bit; a
+ * directory without that file restarts as the reference does.
+ *
+ * Return codes as in afid.h; AFID_ERR_IO for a missing or malformed file,
+ * AFID_ERR_GRID when continua_master.h5 names another grid (the reference
+ * would interpolate, ReadFlowField.F90:56-141).
+ */
+#ifndef AFID_IO_H
+#define AFID_IO_H
+#include "afid.h"
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* WriteFlowField.F90:11-45 into directory dir (time from afid_get_time). */
+int afid_write_continuation(afid_ctx *ctx, const char *dir);
+/* ReadFlowField.F90:29-45, 160-170 from directory dir: fields, time
+ * (afid_set_time), and the state file when present. */
+int afid_read_continuation(afid_ctx *ctx, const char *dir);
+
+/* One FP64 dataset of Fortran shape (n1, n2, n3), column-major host array
+ * (HdfWriteRealHalo3D / HdfReadRealHalo3D, HdfRoutines.F90:299-477, serial). */
+int afid_h5_write_field(const char *path, const char *dset, const double *host, int n1, int n2, int n3);
+int afid_h5_read_field(const char *path, const char *dset, double *host, int n1, int n2, int n3);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

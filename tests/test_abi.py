@@ -28,6 +28,16 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_io_library_exports_every_declared_symbol():
+    if not os.path.exists(engine.IO_LIB_PATH):
+        pytest.skip("libafid_io.so not built")
+    lib = ctypes.CDLL(engine.IO_LIB_PATH)
+    names = engine.exported_symbols("afid_io.h")
+    assert {"afid_write_continuation", "afid_read_continuation"} <= set(names)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
 def test_strerror_without_device():
     if not os.path.exists(engine.LIB_PATH):
         pytest.skip("libafid_hip.so not built")
