@@ -530,21 +530,26 @@ __global__ void __launch_bounds__(WG)
 
 // ---------------------------------------------------------------- split update
 // The same substep update as k_fused_update in two passes.
-// k_stencil_rhs: one block per column (strip order, XCD-remapped, so the y/z
-// neighbour columns it reads were just read by nearby blocks of the same XCD);
-// each thread takes stored points of the column and computes the explicit
-// terms and right-hand sides of all four fields (ExplicitTerms*,
-// ImplicitAndUpdate* RHS loops), updates the histories, and stores
+// k_stencil_rhs: the explicit terms and right-hand sides of all four fields
+// (ExplicitTerms*, ImplicitAndUpdate* RHS loops), the history update, and
 // b' = ackl_b*rhs + A q_old, where A is the unit-diagonal normalised system of
-// SolveImpEqnUpdate_*; identity rows store q_old (+ rhs(1) = the stale row for vx).
-// Every store is unconditional (selects instead of branches) so that all loads
-// of a point issue in one batch; the history rows outside the reference loops
-// (k = 1 and nx for vx/temp, nx for vy/vz), which nothing reads, are kept at 0.
+// SolveImpEqnUpdate_*; identity rows store q_old (+ rhs(1) = the stale row for
+// vx).  A block owns up to 256 stored positions of one y column j and marches
+// through ZM consecutive z planes: each thread keeps its position's values of
+// planes i-1, i and i+1 in registers, so the z neighbours cost no loads and
+// each field value is fetched from L2 once for the centre column and once for
+// each y neighbour.  Blocks are ordered z-range major, then y (XCD-remapped),
+// so the y neighbour columns are being read by neighbouring blocks of the same
+// XCD at the same time.  Every store is unconditional (selects instead of
+// branches) so that a plane's loads issue in one batch; the history rows
+// outside the reference loops (k = 1 and nx for vx/temp, nx for vy/vz), which
+// nothing reads, are kept at 0.
 // k_column_solve: a wave owns four columns (16 chunk lanes each) and solves the
 // four systems in registers: A q_new = b', with the factor rows staged in LDS.
 // The vz increment at k=1, q_new - q_old, becomes the new stale row and the
 // temperature's rhs(1).
-constexpr int STENCIL_MAX_WG = 512;
+constexpr int STENCIL_WG = 256;
+constexpr int ZM = 16;   // z planes per block
 // Coefficient rows staged in LDS by k_stencil_rhs: eight per-k metric rows,
 // then scale, dl, du of the x, y/z and temperature systems.
 enum StencilCoef {
@@ -556,68 +561,93 @@ __device__ __forceinline__ int kc_row_of(int r) {
   return rows[r];
 }
 __device__ __forceinline__ int fac_row_of(int q) { return q == 0 ? FR_SCALE : (q == 1 ? FR_DL : FR_DU); }
-static size_t stencil_lds(const Geom &g) { return sizeof(double) * SC_N * (size_t)g.ldx; }
+static size_t stencil_lds(const Geom &) { return sizeof(double) * SC_N * STENCIL_WG; }
+
+// The centre-column values of one plane at a thread's position (k-1, k, k+1).
+struct Plane {
+  double vxm, vxc, vxp, vym, vyc, vyp, vzm, vzc, vzp, tm, tc, tp, prm, prc;
+};
+template <typename OffT>
+__device__ __forceinline__ Plane load_plane(const double *__restrict__ vx, const double *__restrict__ vy,
+                                            const double *__restrict__ vz, const double *__restrict__ T,
+                                            const double *__restrict__ pr, OffT C, OffT smb, OffT sb, OffT spb) {
+  Plane q;
+  q.vxm = ldf(vx, C + smb); q.vxc = ldf(vx, C + sb); q.vxp = ldf(vx, C + spb);
+  q.vym = ldf(vy, C + smb); q.vyc = ldf(vy, C + sb); q.vyp = ldf(vy, C + spb);
+  q.vzm = ldf(vz, C + smb); q.vzc = ldf(vz, C + sb); q.vzp = ldf(vz, C + spb);
+  q.tm = ldf(T, C + smb); q.tc = ldf(T, C + sb); q.tp = ldf(T, C + spb);
+  q.prm = ldf(pr, C + smb); q.prc = ldf(pr, C + sb);
+  return q;
+}
+
 template <typename OffT, bool USE_RU>
-__global__ void __launch_bounds__(STENCIL_MAX_WG)
+__global__ void __launch_bounds__(STENCIL_WG)
     k_stencil_rhs(FusedArgs a, const double *__restrict__ vx, const double *__restrict__ vy,
                   const double *__restrict__ vz, const double *__restrict__ T, const double *__restrict__ pr,
                   double *__restrict__ rux, double *__restrict__ ruy, double *__restrict__ ruz,
                   double *__restrict__ rut, double *__restrict__ bx, double *__restrict__ by,
                   double *__restrict__ bz, double *__restrict__ bt, const double *__restrict__ stalep,
                   const double *__restrict__ kc, const double *__restrict__ facp) {
-  extern __shared__ double cf[];   // [SC_N][ldx] per-k coefficient rows
+  extern __shared__ double cf[];   // [SC_N][STENCIL_WG] per-k coefficient rows of the block's positions
   const Geom g = a.g;
-  const int nstrip = (g.nym + STRIP_W - 1) / STRIP_W;
-  const int64_t row = xcd_block((int64_t)nstrip * g.nzm);   // (strip, z) rows, z fastest
-  if (row < 0) return;
-  const int strip = (int)(row / g.nzm), ic = (int)(row - (int64_t)strip * g.nzm);
   const int L = g.L, ldx = g.ldx, nx = g.nx, nxm = g.nxm;
-  // the coefficient rows depend on k only: stage them once for the block's columns
-  for (int idx = threadIdx.x; idx < SC_N * ldx; idx += blockDim.x) {
-    const int r = idx / ldx, s = idx - r * ldx;
+  const int nparts = (ldx + STENCIL_WG - 1) / STENCIL_WG;
+  const int nib = (g.nzm + ZM - 1) / ZM;
+  const int64_t tile = xcd_block((int64_t)nib * g.nym * nparts);
+  if (tile < 0) return;
+  const int part = (int)(tile % nparts);
+  const int64_t rest = tile / nparts;
+  const int jc = (int)(rest % g.nym), ib = (int)(rest / g.nym);
+  const int s = part * STENCIL_WG + (int)threadIdx.x;
+  for (int idx = threadIdx.x; idx < SC_N * STENCIL_WG; idx += STENCIL_WG) {
+    const int r = idx / STENCIL_WG, sl = min(part * STENCIL_WG + idx - r * STENCIL_WG, ldx - 1);
     const int src = r < SC_XSC ? kc_row_of(r) * ldx : (((r - SC_XSC) / 3) * FR_N + fac_row_of((r - SC_XSC) % 3)) * ldx;
-    cf[idx] = (r < SC_XSC ? kc : facp)[src + s];
+    cf[idx] = (r < SC_XSC ? kc : facp)[src + sl];
   }
   __syncthreads();
-  const int im = wrapm(ic - 1, g.nzm), ip = wrapp(ic + 1, g.nzm);
+  if (s >= ldx) return;
+  const int k = k_of_pos(s, L);
+  if (k >= nx) return;
+  const int jm = wrapm(jc - 1, g.nym), jp = wrapp(jc + 1, g.nym);
   const OffT cb = (OffT)ldx * sizeof(double);
   auto colb = [&](int jj, int ii) { return ((OffT)ii * (OffT)g.nym + (OffT)jj) * cb; };
-  const int j0 = strip * STRIP_W, j1 = min(j0 + STRIP_W, g.nym);
-  for (int jc = j0; jc < j1; jc++) {
-  const int jm = wrapm(jc - 1, g.nym), jp = wrapp(jc + 1, g.nym);
-  const OffT C = colb(jc, ic), CJM = colb(jm, ic), CJP = colb(jp, ic), CIM = colb(jc, im), CIP = colb(jc, ip),
-             CJMIP = colb(jm, ip), CJPIM = colb(jp, im);
-  const double stale0 = stalep[(int64_t)ic * g.nym + jc];   // stale row of this column
-  for (int s = threadIdx.x; s < ldx; s += blockDim.x) {
-    const int k = k_of_pos(s, L);
-    if (k >= nx) continue;
-    const OffT sb = (OffT)s * sizeof(double);
-    // k-1 / k+1 positions; at the column ends they point at k itself (values unused there)
-    const OffT smb = (OffT)(k >= 1 ? pos_of_k(k - 1, L) : s) * sizeof(double);
-    const OffT spb = (OffT)(k + 1 < nx ? pos_of_k(k + 1, L) : s) * sizeof(double);
-    const bool inx = k >= 1 && k <= nx - 2;   // interior rows of the nx systems
-    const bool iny = k <= nxm - 1;            // rows of the nxm systems
-    const StepScalars &c = a.sc;
-    auto cfv = [&](int r) { return cf[r * ldx + s]; };
-    // centre column, k-1 / k / k+1
-    const double vxm = ldf(vx, C + smb), vxc = ldf(vx, C + sb), vxp = ldf(vx, C + spb);
-    const double vym = ldf(vy, C + smb), vyc = ldf(vy, C + sb), vyp = ldf(vy, C + spb);
-    const double vzm = ldf(vz, C + smb), vzc = ldf(vz, C + sb), vzp = ldf(vz, C + spb);
-    const double tm = ldf(T, C + smb), tc = ldf(T, C + sb), tp = ldf(T, C + spb);
-    const double prm = ldf(pr, C + smb), prc = ldf(pr, C + sb);
-    // neighbour columns at k
-    const double vxjp = ldf(vx, CJP + sb), vxjm = ldf(vx, CJM + sb), vxip = ldf(vx, CIP + sb), vxim = ldf(vx, CIM + sb);
-    const double vyjp = ldf(vy, CJP + sb), vyjm = ldf(vy, CJM + sb), vyip = ldf(vy, CIP + sb), vyim = ldf(vy, CIM + sb);
-    const double vzjp = ldf(vz, CJP + sb), vzjm = ldf(vz, CJM + sb), vzip = ldf(vz, CIP + sb), vzim = ldf(vz, CIM + sb);
-    const double tjp = ldf(T, CJP + sb), tjm = ldf(T, CJM + sb), tip = ldf(T, CIP + sb), tim = ldf(T, CIM + sb);
-    const double prjm = ldf(pr, CJM + sb), prim = ldf(pr, CIM + sb);
-    const double vyjpim = ldf(vy, CJPIM + sb), vzjmip = ldf(vz, CJMIP + sb);
-    // neighbour columns at k-1 / k+1
-    const double vyjpm = ldf(vy, CJP + smb), vzipm = ldf(vz, CIP + smb);
-    const double vxjmp = ldf(vx, CJM + spb), vximp = ldf(vx, CIM + spb);
-    const double udx3c = cfv(SC_UDX3C), udx3m = cfv(SC_UDX3M);
-    const double amc = cfv(SC_AMC), acc = cfv(SC_ACC), apc = cfv(SC_APC);
-    const double ams = cfv(SC_AMS), acs = cfv(SC_ACS), aps = cfv(SC_APS);
+  const OffT sb = (OffT)s * sizeof(double);
+  // k-1 / k+1 positions; at the column ends they point at k itself (values unused there)
+  const OffT smb = (OffT)(k >= 1 ? pos_of_k(k - 1, L) : s) * sizeof(double);
+  const OffT spb = (OffT)(k + 1 < nx ? pos_of_k(k + 1, L) : s) * sizeof(double);
+  const bool inx = k >= 1 && k <= nx - 2;   // interior rows of the nx systems
+  const bool iny = k <= nxm - 1;            // rows of the nxm systems
+  const StepScalars &c = a.sc;
+  const int sl = (int)threadIdx.x;
+  auto cfv = [&](int r) { return cf[r * STENCIL_WG + sl]; };
+  const double udx3c = cfv(SC_UDX3C), udx3m = cfv(SC_UDX3M);
+  const double amc = cfv(SC_AMC), acc = cfv(SC_ACC), apc = cfv(SC_APC);
+  const double ams = cfv(SC_AMS), acs = cfv(SC_ACS), aps = cfv(SC_APS);
+  const int i0 = ib * ZM, i1 = min(i0 + ZM, g.nzm);
+  // prologue: plane i0 and the values of plane i0-1 the marching step needs
+  Plane cur = load_plane<OffT>(vx, vy, vz, T, pr, colb(jc, i0), smb, sb, spb);
+  const int ib0 = wrapm(i0 - 1, g.nzm);
+  const OffT CP = colb(jc, ib0);
+  double vxim = ldf(vx, CP + sb), vximp = ldf(vx, CP + spb), vyim = ldf(vy, CP + sb);
+  double vzim = ldf(vz, CP + sb), tim = ldf(T, CP + sb), prim = ldf(pr, CP + sb);
+  double vyjpim = ldf(vy, colb(jp, ib0) + sb);
+  for (int ic = i0; ic < i1; ic++) {
+    const int ip = wrapp(ic + 1, g.nzm);
+    const OffT C = colb(jc, ic), CJM = colb(jm, ic), CJP = colb(jp, ic);
+    const Plane nxt = load_plane<OffT>(vx, vy, vz, T, pr, colb(jc, ip), smb, sb, spb);
+    const double vxjp = ldf(vx, CJP + sb), vyjp = ldf(vy, CJP + sb), vyjpm = ldf(vy, CJP + smb);
+    const double vzjp = ldf(vz, CJP + sb), tjp = ldf(T, CJP + sb);
+    const double vxjm = ldf(vx, CJM + sb), vxjmp = ldf(vx, CJM + spb), vyjm = ldf(vy, CJM + sb);
+    const double vzjm = ldf(vz, CJM + sb), tjm = ldf(T, CJM + sb), prjm = ldf(pr, CJM + sb);
+    const double vzjmip = ldf(vz, colb(jm, ip) + sb);
+    const double rxo = USE_RU ? ldf(rux, C + sb) : 0.0, ryo = USE_RU ? ldf(ruy, C + sb) : 0.0;
+    const double rzo = USE_RU ? ldf(ruz, C + sb) : 0.0, rto = USE_RU ? ldf(rut, C + sb) : 0.0;
+    const double st0 = stalep[(int64_t)ic * g.nym + jc];   // uniform address
+    const double stale0 = k == 0 ? st0 : 0.0;
+    const double vxm = cur.vxm, vxc = cur.vxc, vxp = cur.vxp, vym = cur.vym, vyc = cur.vyc, vyp = cur.vyp;
+    const double vzm = cur.vzm, vzc = cur.vzc, vzp = cur.vzp, tm = cur.tm, tc = cur.tc, tp = cur.tp;
+    const double prm = cur.prm, prc = cur.prc;
+    const double vxip = nxt.vxc, vyip = nxt.vyc, vzip = nxt.vzc, vzipm = nxt.vzm, tip = nxt.tc;
 
     // ---- vx (ExplicitTermsVX.F90:55-101, ImplicitAndUpdateVX.F90:47-60)
     {
@@ -630,10 +660,10 @@ __global__ void __launch_bounds__(STENCIL_MAX_WG)
       double q = -(hxx + hxy + hxz) + dyyvx + dzzvx + tc;
       double dxxvx = vxp * apc + vxc * acc + vxm * amc;
       double dxp = (prc - prm) * (c.al * udx3c);
-      double r = USE_RU ? c.ro * ldf(rux, C + sb) : 0.0;
+      double r = USE_RU ? c.ro * rxo : 0.0;
       double rhs = (c.ga * q + r + c.alre * dxxvx - dxp) * c.dt;
       double vin = rhs * cfv(SC_XSC) + ((cfv(SC_XDL) * vxm + vxc) + cfv(SC_XDU) * vxp);
-      double vbd = (k == 0 ? stale0 : 0.0) + vxc;
+      double vbd = stale0 + vxc;
       stf(rux, C + sb, inx ? q : 0.0);
       stf(bx, C + sb, inx ? vin : vbd);
     }
@@ -649,7 +679,7 @@ __global__ void __launch_bounds__(STENCIL_MAX_WG)
       double h = -(hyx + hyy + hyz) + dyyvy + dzzvy;
       double dxxvy = vypp * aps + vyc * acs + vymm * ams;
       double dyp = (prc - prjm) * c.udy_al;
-      double r = USE_RU ? c.ro * ldf(ruy, C + sb) : 0.0;
+      double r = USE_RU ? c.ro * ryo : 0.0;
       double rhs = (c.ga * h + r + c.alre * dxxvy - dyp) * c.dt;
       double vin = rhs * cfv(SC_YSC) + ((cfv(SC_YDL) * vym + vyc) + cfv(SC_YDU) * vyp);
       stf(ruy, C + sb, iny ? h : 0.0);
@@ -667,7 +697,7 @@ __global__ void __launch_bounds__(STENCIL_MAX_WG)
       double h = -(hzx + hzy + hzz) + dyyvz + dzzvz;
       double dxxvz = vzpp * aps + vzc * acs + vzmm * ams;
       double dzp = (prc - prim) * c.dz * c.al;
-      double r = USE_RU ? c.ro * ldf(ruz, C + sb) : 0.0;
+      double r = USE_RU ? c.ro * rzo : 0.0;
       double rhs = (c.ga * h + r + c.alre * dxxvz - dzp) * c.dt;
       double vin = rhs * cfv(SC_YSC) + ((cfv(SC_YDL) * vzm + vzc) + cfv(SC_YDU) * vzp);
       stf(ruz, C + sb, iny ? h : 0.0);
@@ -682,13 +712,16 @@ __global__ void __launch_bounds__(STENCIL_MAX_WG)
       double dyyt = (tjp - 2.0 * tc + tjm) * c.udyq_pe;
       double h = -(htx + hty + htz) + dyyt + dzzt;
       double dxxt = tp * apc + tc * acc + tm * amc;
-      double r = USE_RU ? c.ro * ldf(rut, C + sb) : 0.0;
+      double r = USE_RU ? c.ro * rto : 0.0;
       double rhs = (c.ga * h + r + c.alpec * dxxt) * c.dt;
       double vin = rhs * cfv(SC_TSC) + ((cfv(SC_TDL) * tm + tc) + cfv(SC_TDU) * tp);
       stf(rut, C + sb, inx ? h : 0.0);
       stf(bt, C + sb, inx ? vin : tc);   // row 1 gets the vz increment added by the solve pass
     }
-  }
+    // march: plane i becomes i-1, plane i+1 becomes i
+    vxim = vxc; vximp = vxp; vyim = vyc; vzim = vzc; tim = tc; prim = prc;
+    vyjpim = vyjp;
+    cur = nxt;
   }
 }
 
@@ -1097,18 +1130,16 @@ hipError_t launch_fused_update(const FusedArgs &a, hipStream_t st) {
 
 hipError_t launch_stencil_rhs(const FusedArgs &a, hipStream_t st) {
   const int64_t n = a.g.ncol * a.g.ldx;
-  const int64_t nrow = (int64_t)((a.g.nym + STRIP_W - 1) / STRIP_W) * a.g.nzm;
-  const dim3 grid((unsigned)(((nrow + 7) / 8) * 8));
-  // a block walks the STRIP_W columns of one (strip, z) row; one wave per 64
-  // stored points of a column, up to STENCIL_MAX_WG
-  const dim3 block((unsigned)std::min(STENCIL_MAX_WG, (a.g.ldx + 63) / 64 * 64));
+  const int64_t ntile = (int64_t)((a.g.nzm + ZM - 1) / ZM) * a.g.nym * ((a.g.ldx + STENCIL_WG - 1) / STENCIL_WG);
+  const dim3 grid((unsigned)(((ntile + 7) / 8) * 8));
   const size_t shm = stencil_lds(a.g);
   const bool small = (uint64_t)n * sizeof(double) < (1ull << 32);
   const bool ru = a.sc.use_ru != 0;
 #define SR(O, R)                                                                                         \
   if (std::is_same<O, uint32_t>::value == small && R == ru) {                                          \
-    hipLaunchKernelGGL((k_stencil_rhs<O, R>), grid, block, shm, st, a, a.f.vx, a.f.vy, a.f.vz, a.f.temp,   \
-                       a.f.pr, a.rux, a.ruy, a.ruz, a.rut, a.bx, a.by, a.bz, a.bt, a.stale, a.kc, a.fac);   \
+    hipLaunchKernelGGL((k_stencil_rhs<O, R>), grid, dim3(STENCIL_WG), shm, st, a, a.f.vx, a.f.vy, a.f.vz, \
+                       a.f.temp, a.f.pr, a.rux, a.ruy, a.ruz, a.rut, a.bx, a.by, a.bz, a.bt, a.stale, a.kc, \
+                       a.fac);                                                                         \
     return hipGetLastError();                                                                          \
   }
   SR(uint32_t, true) SR(uint32_t, false) SR(uint64_t, true) SR(uint64_t, false)
@@ -1168,9 +1199,9 @@ hipError_t launch_correct(const CorrectArgs &a, int which, hipStream_t st) {
   return hipGetLastError();
 }
 
-// LDS per block: WG*L doubles (fused update), SC_N*16*L (stencil pass),
-// SYS_N*RW_N*16*L (column solve); L <= 72 keeps all within gfx950's 160 KiB.
-int max_supported_L() { return 72; }
+// LDS per block: WG*L doubles (fused update), SC_N*STENCIL_WG (stencil pass),
+// SYS_N*RW_N*16*L (column solve); L <= 80 keeps all within gfx950's 160 KiB.
+int max_supported_L() { return 80; }
 size_t column_lds_need(const Geom &g) { return std::max(stencil_lds(g), solve_lds(g)); }
 
 }  // namespace afid

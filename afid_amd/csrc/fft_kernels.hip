@@ -43,12 +43,20 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// In-place DIT stages over `nlines` lines of length pl.n stored contiguously
-// in LDS from `lines`, input already in digit-reversed order, computed by ONE
-// wavefront (lane = 0..63): the stages synchronise within the wave only, so
-// the waves of a block progress independently.  sign = -1 forward, +1 backward.
-// Stage with radix r and span m combines, for every block b and offset u < m,
-// the r points b*m*r + u + v*m (v < r) after twiddling point v by W_{mr}^{uv}.
+// Global loads below are issued in batches of LB (the whole batch is in
+// flight before the first value is used) so that a thread never waits for one
+// load at a time.
+constexpr int LB = 8;
+constexpr int TB = 4;   // butterfly iterations per twiddle batch
+
+// In-place DIT stages over `nlines` (1 or 2) lines of length pl.n stored
+// contiguously in LDS from `lines`, input already in digit-reversed order,
+// computed by ONE wavefront (lane = 0..63): the stages synchronise within the
+// wave only, so the waves of a block progress independently.  sign = -1
+// forward, +1 backward.  Stage with radix r and span m combines, for every
+// block b and offset u < m, the r points b*m*r + u + v*m (v < r) after
+// twiddling point v by W_{mr}^{uv}.  The twiddles of TB butterflies per lane
+// are loaded together before any of them is computed.
 __device__ __forceinline__ void lds_fft_wave(double2 *lines, int nlines, const FFTPlan &pl, double sign,
                                              int lane) {
   const int n = pl.n;
@@ -61,63 +69,80 @@ __device__ __forceinline__ void lds_fft_wave(double2 *lines, int nlines, const F
     const int nb = n / r;
     const int tws = n / M;
     const unsigned mdiv = pl.mdiv[st];
-    for (int q = 0; q < nlines; q++) {
-      double2 *line = lines + q * n;
-      for (int w = lane; w < nb; w += 64) {
+    const int total = nlines * nb;
+    for (int w0 = 0; w0 < total; w0 += 64 * TB) {
+      double2 t1[TB], t2[TB], t3[TB], t4[TB];
+      int lo[TB], e0[TB];
+      bool ok[TB];
+#pragma unroll
+      for (int b = 0; b < TB; b++) {
+        const int wid = w0 + b * 64 + lane;
+        ok[b] = wid < total;
+        const int wl = ok[b] ? wid : 0;
+        const int q = wl >= nb ? 1 : 0;
+        const int w = wl - q * nb;
         const int bb = m == 1 ? w : (int)__umulhi((unsigned)w, mdiv);
         const int u = w - bb * m;
-        const int e0 = bb * M + u;
-        double2 *base = line;
-        const int t1i = u * tws;
-        // element v of the butterfly: base[X(v)], X(v) = swz(e0 + v*m)
-#define X(v) swz(e0 + (v) * m, sw)
+        lo[b] = q * n;
+        e0[b] = bb * M + u;
+        const int t1i = u * tws;   // v*t1i < n for v < r; clamp the unused ones
+        t1[b] = twid(tw, t1i, sign);
+        t2[b] = twid(tw, min(2 * t1i, n - 1), sign);
+        t3[b] = twid(tw, min(3 * t1i, n - 1), sign);
+        t4[b] = twid(tw, min(4 * t1i, n - 1), sign);
+      }
+#pragma unroll
+      for (int b = 0; b < TB; b++) {
+        if (!ok[b]) continue;
+        double2 *line = lines + lo[b];
+#define X(v) line[swz(e0[b] + (v) * m, sw)]
         if (r == 2) {
-          double2 x0 = base[X(0)], x1 = cmul(base[X(1)], twid(tw, t1i, sign));
-          base[X(0)] = cadd(x0, x1);
-          base[X(1)] = csub(x0, x1);
+          double2 x0 = X(0), x1 = cmul(X(1), t1[b]);
+          X(0) = cadd(x0, x1);
+          X(1) = csub(x0, x1);
         } else if (r == 4) {
-          double2 x0 = base[X(0)];
-          double2 x1 = cmul(base[X(1)], twid(tw, t1i, sign));
-          double2 x2 = cmul(base[X(2)], twid(tw, 2 * t1i, sign));
-          double2 x3 = cmul(base[X(3)], twid(tw, 3 * t1i, sign));
+          double2 x0 = X(0);
+          double2 x1 = cmul(X(1), t1[b]);
+          double2 x2 = cmul(X(2), t2[b]);
+          double2 x3 = cmul(X(3), t3[b]);
           double2 a0 = cadd(x0, x2), a1 = csub(x0, x2);
           double2 b0 = cadd(x1, x3), b1 = mul_si(csub(x1, x3), sign);
-          base[X(0)] = cadd(a0, b0);
-          base[X(1)] = cadd(a1, b1);
-          base[X(2)] = csub(a0, b0);
-          base[X(3)] = csub(a1, b1);
+          X(0) = cadd(a0, b0);
+          X(1) = cadd(a1, b1);
+          X(2) = csub(a0, b0);
+          X(3) = csub(a1, b1);
         } else if (r == 3) {
           const double s3 = 0.86602540378443864676372317075294;  // sin(2 pi/3)
-          double2 x0 = base[X(0)];
-          double2 x1 = cmul(base[X(1)], twid(tw, t1i, sign));
-          double2 x2 = cmul(base[X(2)], twid(tw, 2 * t1i, sign));
-          double2 t1 = cadd(x1, x2);
-          double2 t2 = mul_si(csub(x1, x2), sign * s3);
-          double2 a = make_double2(x0.x - 0.5 * t1.x, x0.y - 0.5 * t1.y);
-          base[X(0)] = cadd(x0, t1);
-          base[X(1)] = cadd(a, t2);
-          base[X(2)] = csub(a, t2);
+          double2 x0 = X(0);
+          double2 x1 = cmul(X(1), t1[b]);
+          double2 x2 = cmul(X(2), t2[b]);
+          double2 p1 = cadd(x1, x2);
+          double2 p2 = mul_si(csub(x1, x2), sign * s3);
+          double2 q0 = make_double2(x0.x - 0.5 * p1.x, x0.y - 0.5 * p1.y);
+          X(0) = cadd(x0, p1);
+          X(1) = cadd(q0, p2);
+          X(2) = csub(q0, p2);
         } else {  // r == 5
           const double c1 = 0.30901699437494742410229341718282;   // cos(2 pi/5)
           const double c2 = -0.80901699437494742410229341718282;  // cos(4 pi/5)
           const double s1 = 0.95105651629515357211643933337938;   // sin(2 pi/5)
           const double s2 = 0.58778525229247312916870595463907;   // sin(4 pi/5)
-          double2 x0 = base[X(0)];
-          double2 x1 = cmul(base[X(1)], twid(tw, t1i, sign));
-          double2 x2 = cmul(base[X(2)], twid(tw, 2 * t1i, sign));
-          double2 x3 = cmul(base[X(3)], twid(tw, 3 * t1i, sign));
-          double2 x4 = cmul(base[X(4)], twid(tw, 4 * t1i, sign));
-          double2 t1 = cadd(x1, x4), t2 = cadd(x2, x3);
-          double2 t3 = csub(x1, x4), t4 = csub(x2, x3);
-          double2 a1 = make_double2(x0.x + c1 * t1.x + c2 * t2.x, x0.y + c1 * t1.y + c2 * t2.y);
-          double2 a2 = make_double2(x0.x + c2 * t1.x + c1 * t2.x, x0.y + c2 * t1.y + c1 * t2.y);
-          double2 b1 = mul_si(make_double2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y), sign);
-          double2 b2 = mul_si(make_double2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y), sign);
-          base[X(0)] = cadd(x0, cadd(t1, t2));
-          base[X(1)] = cadd(a1, b1);
-          base[X(2)] = cadd(a2, b2);
-          base[X(3)] = csub(a2, b2);
-          base[X(4)] = csub(a1, b1);
+          double2 x0 = X(0);
+          double2 x1 = cmul(X(1), t1[b]);
+          double2 x2 = cmul(X(2), t2[b]);
+          double2 x3 = cmul(X(3), t3[b]);
+          double2 x4 = cmul(X(4), t4[b]);
+          double2 p1 = cadd(x1, x4), p2 = cadd(x2, x3);
+          double2 p3 = csub(x1, x4), p4 = csub(x2, x3);
+          double2 a1 = make_double2(x0.x + c1 * p1.x + c2 * p2.x, x0.y + c1 * p1.y + c2 * p2.y);
+          double2 a2 = make_double2(x0.x + c2 * p1.x + c1 * p2.x, x0.y + c2 * p1.y + c1 * p2.y);
+          double2 b1 = mul_si(make_double2(s1 * p3.x + s2 * p4.x, s1 * p3.y + s2 * p4.y), sign);
+          double2 b2 = mul_si(make_double2(s2 * p3.x - s1 * p4.x, s2 * p3.y - s1 * p4.y), sign);
+          X(0) = cadd(x0, cadd(p1, p2));
+          X(1) = cadd(a1, b1);
+          X(2) = cadd(a2, b2);
+          X(3) = csub(a2, b2);
+          X(4) = csub(a1, b1);
         }
 #undef X
       }
@@ -142,7 +167,8 @@ constexpr int TM = 4;   // complex lines (y wavenumbers) per z-FFT block
 
 // ---------------------------------------------------------------- y forward
 // tile: (TK consecutive interleaved positions, z index i); wave w transforms
-// lines 2w and 2w+1 of the tile
+// lines 2w and 2w+1 of the tile.  Thread t loads slot q = t % TK of the rows
+// j = t / TK + 32 it, so its k, neighbour slot and udx3m stay fixed.
 __global__ void __launch_bounds__(256) k_yfft_forward(PoissonArgs a) {
   extern __shared__ double2 lds[];
   const Geom g = a.g;
@@ -154,50 +180,82 @@ __global__ void __launch_bounds__(256) k_yfft_forward(PoissonArgs a) {
   const int s0 = (int)(tile % ntile) * TK, i = (int)(tile / ntile);
   const bool hsw = (H & 7) == 0;
   const double usdtal = 1.0 / (a.sc.dt * a.sc.al);
-  for (int idx = threadIdx.x; idx < TK * N; idx += blockDim.x) {
-    const int q = idx % TK, j = idx / TK;
+  {
+    const int q = threadIdx.x % TK, jr = threadIdx.x / TK;   // 32 rows per pass
     const int s = s0 + q, k = k_of_pos(s, g.L);
-    double v = 0.0;
-    if (k < g.nxm) {
-      const int64_t C = col_off(j, i, g);
-      if (a.from_velocity) {
-        const int64_t CJP = col_off(wrapp(j + 1, g.nym), i, g), CIP = col_off(j, wrapp(i + 1, g.nzm), g);
-        const int skp = pos_of_k(k + 1, g.L);
-        double dq = (a.f.vz[CIP + s] - a.f.vz[C + s]) * a.sc.dz + (a.f.vy[CJP + s] - a.f.vy[C + s]) * a.sc.dy +
-                    (a.f.vx[C + skp] - a.f.vx[C + s]) * a.udx3m[s];
-        v = dq * usdtal;
-      } else {
-        v = a.dph[C + s];
+    const bool live = k < g.nxm;
+    const int skp = live ? pos_of_k(k + 1, g.L) : s;
+    const double u3 = a.udx3m[s];
+    const int ip = wrapp(i + 1, g.nzm);
+    constexpr int YB = 4;   // rows per batch: 6 field loads each, all issued before any is used
+    for (int jb = jr; jb < N; jb += 32 * YB) {
+      double z0[YB], z1[YB], y0[YB], y1[YB], x0[YB], x1[YB];
+      int rv[YB];
+#pragma unroll
+      for (int b = 0; b < YB; b++) {
+        const int jc = min(jb + 32 * b, N - 1);
+        const int64_t C = col_off(jc, i, g);
+        if (a.from_velocity) {
+          const int64_t CJP = col_off(wrapp(jc + 1, g.nym), i, g), CIP = col_off(jc, ip, g);
+          z1[b] = a.f.vz[CIP + s];
+          z0[b] = a.f.vz[C + s];
+          y1[b] = a.f.vy[CJP + s];
+          y0[b] = a.f.vy[C + s];
+          x1[b] = a.f.vx[C + skp];
+          x0[b] = a.f.vx[C + s];
+        } else {
+          z0[b] = a.dph[C + s];
+          z1[b] = y0[b] = y1[b] = x0[b] = x1[b] = 0.0;
+        }
+        rv[b] = py.half ? py.c.rev[jc >> 1] : py.c.rev[jc];
       }
-    }
-    if (py.half) {
-      double2 *e = &lds[q * H + swz(py.c.rev[j >> 1], hsw)];
-      if (j & 1) e->y = v;
-      else e->x = v;
-    } else {
-      lds[q * H + swz(py.c.rev[j], hsw)] = make_double2(v, 0.0);
+#pragma unroll
+      for (int b = 0; b < YB; b++) {
+        const int j = jb + 32 * b;
+        if (j >= N) continue;
+        double x = z0[b];
+        if (a.from_velocity)
+          x = ((z1[b] - z0[b]) * a.sc.dz + (y1[b] - y0[b]) * a.sc.dy + (x1[b] - x0[b]) * u3) * usdtal;
+        const double v = live ? x : 0.0;
+        double2 *e = &lds[q * H + swz(rv[b], hsw)];
+        if (!py.half) *e = make_double2(v, 0.0);
+        else if (j & 1) e->y = v;
+        else e->x = v;
+      }
     }
   }
   __syncthreads();
   lds_fft_wave(lds + (threadIdx.x / 64) * (TK / 4) * H, TK / 4, py.c, -1.0, threadIdx.x % 64);
   __syncthreads();
   const int nymh = a.nymh;
-  for (int idx = threadIdx.x; idx < TK * nymh; idx += blockDim.x) {
-    const int q = idx / nymh, m = idx - q * nymh;
-    const int k = k_of_pos(s0 + q, g.L);
-    if (k >= g.nxm) continue;
-    double2 X;
-    if (py.half) {
-      double2 zm = lds[q * H + swz(m == H ? 0 : m, hsw)];
-      double2 zc = conj2(lds[q * H + swz(m == 0 ? 0 : H - m, hsw)]);
-      double2 E = make_double2(0.5 * (zm.x + zc.x), 0.5 * (zm.y + zc.y));
-      double2 D = csub(zm, zc);
-      double2 O = make_double2(0.5 * D.y, -0.5 * D.x);  // D / (2i)
-      X = cadd(E, cmul(py.post[m], O));
-    } else {
-      X = lds[q * H + swz(m, hsw)];
+  for (int base = threadIdx.x; base < TK * nymh; base += 256 * LB) {
+    double2 pw[LB];
+#pragma unroll
+    for (int b = 0; b < LB; b++) {
+      const int idx = min(base + 256 * b, TK * nymh - 1);
+      const int m = idx % nymh;
+      pw[b] = py.half ? py.post[m] : make_double2(1.0, 0.0);
     }
-    a.S[((int64_t)k * g.nzm + i) * a.ldm + m] = X;
+#pragma unroll
+    for (int b = 0; b < LB; b++) {
+      const int idx = base + 256 * b;
+      if (idx >= TK * nymh) continue;
+      const int q = idx / nymh, m = idx - q * nymh;
+      const int k = k_of_pos(s0 + q, g.L);
+      if (k >= g.nxm) continue;
+      double2 X;
+      if (py.half) {
+        double2 zm = lds[q * H + swz(m == H ? 0 : m, hsw)];
+        double2 zc = conj2(lds[q * H + swz(m == 0 ? 0 : H - m, hsw)]);
+        double2 E = make_double2(0.5 * (zm.x + zc.x), 0.5 * (zm.y + zc.y));
+        double2 D = csub(zm, zc);
+        double2 O = make_double2(0.5 * D.y, -0.5 * D.x);  // D / (2i)
+        X = cadd(E, cmul(pw[b], O));
+      } else {
+        X = lds[q * H + swz(m, hsw)];
+      }
+      a.S[((int64_t)k * g.nzm + i) * a.ldm + m] = X;
+    }
   }
 }
 
@@ -212,17 +270,29 @@ __global__ void __launch_bounds__(256) k_zfft(PoissonArgs a, double sign) {
   const int m0 = (int)(tile % ntile) * TM, k = (int)(tile / ntile);
   const bool zsw = (N & 7) == 0;
   double2 *S = a.S + (int64_t)k * a.g.nzm * a.ldm;
-  for (int idx = threadIdx.x; idx < TM * N; idx += blockDim.x) {
-    const int q = idx % TM, nn = idx / TM;
-    const int m = m0 + q;
-    double2 v = make_double2(0.0, 0.0);
-    if (m < a.nymh) v = S[(int64_t)nn * a.ldm + m];
-    lds[q * N + swz(a.pz.rev[nn], zsw)] = v;
+  const int tot = TM * N;
+  for (int base = threadIdx.x; base < tot; base += 256 * LB) {
+    double2 v[LB];
+    int rv[LB];
+#pragma unroll
+    for (int b = 0; b < LB; b++) {   // all loads of the batch first
+      const int ic = min(base + 256 * b, tot - 1);
+      const int q = ic % TM, nn = ic / TM, m = m0 + q;
+      v[b] = S[(int64_t)nn * a.ldm + min(m, a.nymh - 1)];
+      rv[b] = a.pz.rev[nn];
+    }
+#pragma unroll
+    for (int b = 0; b < LB; b++) {
+      const int idx = base + 256 * b;
+      if (idx >= tot) continue;
+      const int q = idx % TM;
+      lds[q * N + swz(rv[b], zsw)] = m0 + q < a.nymh ? v[b] : make_double2(0.0, 0.0);
+    }
   }
   __syncthreads();
   lds_fft_wave(lds + (threadIdx.x / 64) * N, 1, a.pz, sign, threadIdx.x % 64);
   __syncthreads();
-  for (int idx = threadIdx.x; idx < TM * N; idx += blockDim.x) {
+  for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
     const int q = idx % TM, nn = idx / TM;
     const int m = m0 + q;
     if (m < a.nymh) S[(int64_t)nn * a.ldm + m] = lds[q * N + swz(nn, zsw)];
@@ -235,6 +305,9 @@ __global__ void __launch_bounds__(256) k_zfft(PoissonArgs a, double sign) {
 //   acphT_b = 1/(acphk(k) - ak2(m) - ak1(n)), amph = amphk*acphT_b, apph = apphk*acphT_b.
 // The (0,0) mode is singular (zero row sums); its last pivot is left out and
 // x(nxm) := 0 fixes the additive constant that the reference leaves to rounding.
+// The recurrences walk k in blocks of XB rows whose spectrum loads are issued
+// together before the block's rows are eliminated.
+constexpr int XB = 8;
 __global__ void __launch_bounds__(256) k_xsolve_pressure(PoissonArgs a) {
   const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int nymh = a.nymh, nzm = a.g.nzm, nxm = a.g.nxm;
@@ -249,32 +322,51 @@ __global__ void __launch_bounds__(256) k_xsolve_pressure(PoissonArgs a) {
   const bool zero_mode = (m == 0 && n == 0);
   double u_prev = 1.0, du_prev = 0.0;
   double2 y_prev = make_double2(0.0, 0.0);
-  for (int k = 0; k < nxm; k++) {
-    const double b = 1.0 / (a.acphk[k] - lam - ak1n);
-    const double dl = a.amphk[k] * b, du = a.apphk[k] * b;
-    double2 r = S[k * stride];
-    r.x = r.x / nrm * b;
-    r.y = r.y / nrm * b;
-    double u = 1.0;
-    if (k > 0) {
-      const double l = dl / u_prev;
-      u = 1.0 - l * du_prev;
-      r.x = r.x - l * y_prev.x;
-      r.y = r.y - l * y_prev.y;
+  for (int k0 = 0; k0 < nxm; k0 += XB) {
+    double2 rb[XB];
+#pragma unroll
+    for (int t = 0; t < XB; t++) rb[t] = S[min(k0 + t, nxm - 1) * stride];
+#pragma unroll
+    for (int t = 0; t < XB; t++) {
+      const int k = k0 + t;
+      if (k >= nxm) break;
+      const double b = 1.0 / (a.acphk[k] - lam - ak1n);
+      const double dl = a.amphk[k] * b, du = a.apphk[k] * b;
+      double2 r = rb[t];
+      r.x = r.x / nrm * b;
+      r.y = r.y / nrm * b;
+      double u = 1.0;
+      if (k > 0) {
+        const double l = dl / u_prev;
+        u = 1.0 - l * du_prev;
+        r.x = r.x - l * y_prev.x;
+        r.y = r.y - l * y_prev.y;
+      }
+      y_prev = r;
+      u_prev = u;
+      du_prev = du;
+      double ui = (zero_mode && k == nxm - 1) ? 0.0 : 1.0 / u;
+      S[k * stride] = make_double2(r.x * ui, r.y * ui);
+      cp[k * stride] = du * ui;
     }
-    y_prev = r;
-    u_prev = u;
-    du_prev = du;
-    double ui = (zero_mode && k == nxm - 1) ? 0.0 : 1.0 / u;
-    S[k * stride] = make_double2(r.x * ui, r.y * ui);
-    cp[k * stride] = du * ui;
   }
   double2 x = S[(nxm - 1) * stride];
-  for (int k = nxm - 2; k >= 0; k--) {
-    double2 yk = S[k * stride];
-    const double c = cp[k * stride];
-    x = make_double2(yk.x - c * x.x, yk.y - c * x.y);
-    S[k * stride] = x;
+  for (int k1 = nxm - 2; k1 >= 0; k1 -= XB) {
+    double2 yb[XB];
+    double cb[XB];
+#pragma unroll
+    for (int t = 0; t < XB; t++) {
+      const int k = max(k1 - t, 0);
+      yb[t] = S[k * stride];
+      cb[t] = cp[k * stride];
+    }
+#pragma unroll
+    for (int t = 0; t < XB; t++) {
+      const int k = k1 - t;
+      if (k < 0) break;
+      x = make_double2(yb[t].x - cb[t] * x.x, yb[t].y - cb[t] * x.y);
+      S[k * stride] = x;
+    }
   }
 }
 
@@ -290,29 +382,50 @@ __global__ void __launch_bounds__(256) k_yfft_inverse(PoissonArgs a) {
   const int s0 = (int)(tile % ntile) * TK, i = (int)(tile / ntile);
   const bool hsw = (H & 7) == 0;
   const int nymh = a.nymh;
-  for (int idx = threadIdx.x; idx < TK * H; idx += blockDim.x) {
-    const int q = idx / H, t = idx - q * H;
-    const int k = k_of_pos(s0 + q, g.L);
-    double2 z = make_double2(0.0, 0.0);
-    if (k < g.nxm) {
+  for (int base = threadIdx.x; base < TK * H; base += 256 * LB) {
+    double2 xa[LB], xb[LB], pw[LB];
+    int rv[LB];
+#pragma unroll
+    for (int b = 0; b < LB; b++) {   // all loads of the batch first
+      const int ic = min(base + 256 * b, TK * H - 1);
+      const int q = ic / H, t = ic - q * H;
+      const int k = min(k_of_pos(s0 + q, g.L), g.nxm - 1);
       const double2 *X = a.S + ((int64_t)k * g.nzm + i) * a.ldm;
       if (py.half) {
-        double2 xm = X[t], xh = X[H - t];
+        xa[b] = X[t];
+        xb[b] = X[H - t];
+        pw[b] = py.post[t];
+      } else {
+        xa[b] = X[min(t, nymh - 1)];
+        xb[b] = X[min(N - t, nymh - 1)];
+        pw[b] = make_double2(1.0, 0.0);
+      }
+      rv[b] = py.c.rev[t];
+    }
+#pragma unroll
+    for (int b = 0; b < LB; b++) {
+      const int idx = base + 256 * b;
+      if (idx >= TK * H) continue;
+      const int q = idx / H, t = idx - q * H;
+      const bool live = k_of_pos(s0 + q, g.L) < g.nxm;
+      double2 z;
+      if (py.half) {
+        double2 xm = xa[b], xh = xb[b];
         if (t == 0) { xm.y = 0.0; xh.y = 0.0; }  // DC and Nyquist: imaginary parts ignored
         double2 xc = conj2(xh);
         double2 E = cadd(xm, xc);
-        double2 O = cmul(csub(xm, xc), conj2(py.post[t]));
+        double2 O = cmul(csub(xm, xc), conj2(pw[b]));
         z = make_double2(E.x - O.y, E.y + O.x);  // E + i O
       } else {
         if (t < nymh) {
-          z = X[t];
+          z = xa[b];
           if (t == 0) z.y = 0.0;
         } else {
-          z = conj2(X[N - t]);
+          z = conj2(xb[b]);
         }
       }
+      lds[q * H + swz(rv[b], hsw)] = live ? z : make_double2(0.0, 0.0);
     }
-    lds[q * H + swz(py.c.rev[t], hsw)] = z;
   }
   __syncthreads();
   lds_fft_wave(lds + (threadIdx.x / 64) * (TK / 4) * H, TK / 4, py.c, +1.0, threadIdx.x % 64);
