@@ -31,7 +31,9 @@ HBM_PEAK_GBS = 8000.0
 # traffic: explicit VX 5, VY 4, VZ 4, Temp 5; implicit VX/VY/VZ 6 each, Temp 5;
 # divergence 4; pressure solve 10 (5 passes); CorrectVelocity 7; CorrectPressure 3.
 WORDS = {
-    "stencil_rhs": 9 + 8,
+    # RK3: the first substep (ro = 0) does not read the four histories (13
+    # words), the other two do (17); the HIP-event average covers all three.
+    "stencil_rhs": (13 + 17 + 17) / 3,
     "column_solve": 4 + 4,
     "fused_update": 9 + 4 + 4,
     "div_yfft_r2c": 3 + 1,
@@ -42,6 +44,33 @@ WORDS = {
     "correct_vel_pr": 5 + 4,
 }
 SUBSTEP_WORDS = 65
+
+# Measured HBM bytes per launch (rocprofv3 FETCH_SIZE / WRITE_SIZE scaled by a
+# 1 GiB copy; profiles/substep_kernels_cfg2_384x768x768_pmc.md), per bench
+# kernel name: the PMC kernels it stands for and their weights per substep.
+PMC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_cfg2_384x768x768.json")
+PMC_KERNELS = {
+    "stencil_rhs": [("k_stencil_rhs<u32, F>", 1 / 3), ("k_stencil_rhs<u32, T>", 2 / 3)],
+    "column_solve": [("k_column_solve<25, u32>", 1.0)],
+    "correct_vel_pr": [("k_correct", 1.0)],
+    "div_yfft_r2c": [("k_yfft_forward", 1.0)],
+    "zfft_fwd": [("k_zfft", 1.0)],
+    "zfft_bwd": [("k_zfft", 1.0)],
+    "xsolve_pressure": [("k_xsolve_pressure", 1.0)],
+    "yfft_c2r": [("k_yfft_inverse", 1.0)],
+}
+
+
+def measured_traffic(config: str, kernel: str):
+    """HBM bytes per launch of `kernel` from the committed PMC summary (cfg2 only)."""
+    if config != "cfg2" or kernel not in PMC_KERNELS or not os.path.exists(PMC_FILE):
+        return None
+    with open(PMC_FILE) as f:
+        k = json.load(f)["kernels"]
+    try:
+        return round(sum(w * k[name]["traffic_bytes"] for name, w in PMC_KERNELS[kernel]))
+    except KeyError:
+        return None
 
 CONFIGS = {
     "cfg2": dict(nxm=384, nym=768, nzm=768, istr3=6, str3=6.0, ylen=1.0, zlen=1.0, ray=1e8, pra=1.0, seed=2),
@@ -137,11 +166,11 @@ def main():
     kern = {k: {"avg_ms": v[0] / v[1], "launches": v[1]} for k, v in prof.items() if v[1] > 0}
     substep_ms = sum(v[0] for k, v in prof.items() if k in WORDS) / (3 * args.profile_steps)
     dom = max((k for k in kern if k in WORDS), key=lambda k: kern[k]["avg_ms"] * kern[k]["launches"])
-    alg_bytes = WORDS[dom] * 8 * cells
+    alg_bytes = round(WORDS[dom] * 8 * cells)
     achieved = alg_bytes / (kern[dom]["avg_ms"] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
-                "alg_bytes_per_launch": alg_bytes, "avg_ms": round(kern[dom]["avg_ms"], 4)}
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.config, dom),
+                "kernel": dom, "alg_bytes_per_launch": alg_bytes, "avg_ms": round(kern[dom]["avg_ms"], 4)}
     substep_alg = SUBSTEP_WORDS * 8 * cells / (substep_ms * 1e-3) / 1e9
     cell.close()
 
