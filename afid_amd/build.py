@@ -18,7 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libafid_hip.so")
-SOURCES = ["column_kernels.hip", "fft_kernels.hip", "misc_kernels.hip", "afid_ctx.hip"]
+SOURCES = ["column_kernels.hip", "fft_kernels.hip", "misc_kernels.hip", "step_kernels.hip", "afid_ctx.hip"]
 ARCH = os.environ.get("AFID_OFFLOAD_ARCH", "gfx950")
 FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -48,7 +48,16 @@ struct afid_ctx {
   double *f[AFID_NFIELDS] = {nullptr};
   double *fn[4] = {nullptr};   // second buffers of vx, vy, vz, temp for the fused update
   double *bp[4] = {nullptr};   // split update: right-hand sides b + A q_old of vx, vy, vz, temp
-  int split = 1;               // 1: stencil pass + column-solve pass, 0: one fused pass
+  int split = 1;               // 1: stencil pass + column-solve pass (fields updated in place),
+                               // 0: one fused pass into the second buffers fn
+  // device step control (afid_steps): step state, factorisation inputs/scratch,
+  // factor rows per substep, and the captured graph of one step
+  DevStep *dstep = nullptr;
+  double *mc = nullptr, *fwork = nullptr;
+  double *fac_dev[3] = {nullptr, nullptr, nullptr};
+  StepConsts kconst{};
+  hipGraphExec_t step_graph = nullptr;
+  bool capturing = false;
   double *stale = nullptr;     // rhs(1,j,i)
   double *kc = nullptr;
   double *fac_x = nullptr;     // per-routine factors (SYS_N*FR_N*ldx), current coefficients
@@ -461,15 +470,16 @@ struct Timed {
   afid_ctx *c;
   const char *name;
   hipEvent_t a = nullptr, b = nullptr;
-  Timed(afid_ctx *cc, const char *n) : c(cc), name(n) {
-    if (c->prof) {
+  bool on;
+  Timed(afid_ctx *cc, const char *n) : c(cc), name(n), on(cc->prof && !cc->capturing) {
+    if (on) {
       hipEventCreate(&a);
       hipEventCreate(&b);
       hipEventRecord(a, c->st);
     }
   }
   ~Timed() {
-    if (c->prof) {
+    if (on) {
       hipEventRecord(b, c->st);
       c->pending.push_back({name, a, b});
     }
@@ -484,8 +494,10 @@ int kerr(afid_ctx *c, hipError_t e) {
   return AFID_OK;
 }
 
-PoissonArgs poisson_args(afid_ctx *c, int from_velocity) {
+PoissonArgs poisson_args(afid_ctx *c, int from_velocity, int ns = 0, bool dev = false) {
   PoissonArgs a;
+  a.ds = dev ? c->dstep : nullptr;
+  a.ns = ns;
   a.g = c->g;
   a.nymh = c->nymh;
   a.ldm = c->ldm;
@@ -507,9 +519,9 @@ PoissonArgs poisson_args(afid_ctx *c, int from_velocity) {
   return a;
 }
 
-int run_poisson(afid_ctx *c, int from_velocity) {
+int run_poisson(afid_ctx *c, int from_velocity, int ns = 0, bool dev = false) {
   if (c->singular_mode) return AFID_NUM_SINGULAR;
-  PoissonArgs a = poisson_args(c, from_velocity);
+  PoissonArgs a = poisson_args(c, from_velocity, ns, dev);
   int rc;
   {
     Timed t(c, from_velocity ? "div_yfft_r2c" : "yfft_r2c");
@@ -586,6 +598,11 @@ int afid_destroy(afid_ctx *c) {
   for (int q = 0; q < AFID_NFIELDS; q++) hipFree(c->f[q]);
   for (int q = 0; q < 4; q++) hipFree(c->fn[q]);
   for (int q = 0; q < 4; q++) hipFree(c->bp[q]);
+  for (int q = 0; q < 3; q++) hipFree(c->fac_dev[q]);
+  if (c->step_graph) hipGraphExecDestroy(c->step_graph);
+  hipFree(c->dstep);
+  hipFree(c->mc);
+  hipFree(c->fwork);
   for (int q = 0; q < 4; q++) hipFree(c->fac_sub[q]);
   void *ps[] = {c->stale, c->kc, c->fac_x, c->S, c->cp, c->nat, c->partial, c->out, c->stage,
                 c->dpy.tw, c->dpy.rev, c->dpy.post, c->dpz.tw, c->dpz.rev, c->dpz.post};
@@ -664,10 +681,11 @@ int afid_create(afid_ctx **out, const afid_params *p, const afid_metrics *m) {
   }
   for (int q : {AFID_VX, AFID_VY, AFID_VZ, AFID_TEMP, AFID_PR, AFID_RUX, AFID_RUY, AFID_RUZ, AFID_RUTEMP, AFID_DPH})
     TRY(alloc_field(c, &c->f[q]));
-  for (int q = 0; q < 4; q++) TRY(alloc_field(c, &c->fn[q]));
   if (const char *e = getenv("AFID_UPDATE")) c->split = strcmp(e, "fused") != 0;
   if (c->split)
     for (int q = 0; q < 4; q++) TRY(alloc_field(c, &c->bp[q]));
+  else
+    for (int q = 0; q < 4; q++) TRY(alloc_field(c, &c->fn[q]));
   TRYH(hipMalloc(&c->stale, sizeof(double) * g.ncol));
   TRYH(hipMemsetAsync(c->stale, 0, sizeof(double) * g.ncol, c->st));
   {
@@ -979,9 +997,11 @@ int afid_pressure_spectrum(afid_ctx *c, int stage, double *out) {
   return AFID_OK;
 }
 
-static int correct(afid_ctx *c, int which, const char *name) {
+static int correct(afid_ctx *c, int which, const char *name, int ns = 0, bool dev = false) {
   if (!c) return AFID_ERR_ARG;
   CorrectArgs a;
+  a.ds = dev ? c->dstep : nullptr;
+  a.ns = ns;
   a.g = c->g;
   a.dph = c->f[AFID_DPH];
   a.vx = c->f[AFID_VX]; a.vy = c->f[AFID_VY]; a.vz = c->f[AFID_VZ]; a.pr = c->f[AFID_PR];
@@ -997,24 +1017,28 @@ int afid_update_halo(afid_ctx *c, int field) {
   return AFID_OK;
 }
 
-int afid_substep(afid_ctx *c, int ns, double dt) {
-  if (!c || ns < 1 || ns > c->p.nsst || ns > 3 || dt <= 0) return AFID_ERR_ARG;
-  int rc = afid_set_substep(c, ns, dt);
-  if (rc) return rc;
-  if (c->fac_sub_dt[ns] != dt) {
-    if ((rc = upload_factors(c, c->al, dt, c->fac_sub[ns]))) return rc;
-    c->fac_sub_dt[ns] = dt;
-  }
+// One substep's kernels on the context's stream.  dev = false: coefficients
+// and factor rows from the host (afid_set_substep, upload_factors); dev = true:
+// both come from the device step state (afid_steps), nothing is copied.
+static int enqueue_substep(afid_ctx *c, int ns, bool dev) {
+  int rc;
   FusedArgs a;
   a.g = c->g;
   a.f = fields(c);
-  a.vxn = c->fn[0]; a.vyn = c->fn[1]; a.vzn = c->fn[2]; a.tn = c->fn[3];
+  if (c->split) {   // the column solve reads only b' and vz(k=1): the fields are updated in place
+    a.vxn = c->f[AFID_VX]; a.vyn = c->f[AFID_VY]; a.vzn = c->f[AFID_VZ]; a.tn = c->f[AFID_TEMP];
+  } else {
+    a.vxn = c->fn[0]; a.vyn = c->fn[1]; a.vzn = c->fn[2]; a.tn = c->fn[3];
+  }
   a.rux = c->f[AFID_RUX]; a.ruy = c->f[AFID_RUY]; a.ruz = c->f[AFID_RUZ]; a.rut = c->f[AFID_RUTEMP];
   a.stale = c->stale;
   a.kc = c->kc;
-  a.fac = c->fac_sub[ns];
+  a.fac = dev ? c->fac_dev[ns - 1] : c->fac_sub[ns];
   a.bx = c->bp[0]; a.by = c->bp[1]; a.bz = c->bp[2]; a.bt = c->bp[3];
   a.sc = scalars(c);
+  a.sc.use_ru = c->rom[ns] != 0.0;
+  a.ds = dev ? c->dstep : nullptr;
+  a.ns = ns;
   if (c->split) {
     {
       Timed t(c, "stencil_rhs");
@@ -1023,15 +1047,28 @@ int afid_substep(afid_ctx *c, int ns, double dt) {
     Timed t(c, "column_solve");
     if ((rc = kerr(c, launch_column_solve(a, c->st)))) return rc;
   } else {
-    Timed t(c, "fused_update");
-    if ((rc = kerr(c, launch_fused_update(a, c->st)))) return rc;
+    {
+      Timed t(c, "fused_update");
+      if ((rc = kerr(c, launch_fused_update(a, c->st)))) return rc;
+    }
+    std::swap(c->f[AFID_VX], c->fn[0]);
+    std::swap(c->f[AFID_VY], c->fn[1]);
+    std::swap(c->f[AFID_VZ], c->fn[2]);
+    std::swap(c->f[AFID_TEMP], c->fn[3]);
   }
-  std::swap(c->f[AFID_VX], c->fn[0]);
-  std::swap(c->f[AFID_VY], c->fn[1]);
-  std::swap(c->f[AFID_VZ], c->fn[2]);
-  std::swap(c->f[AFID_TEMP], c->fn[3]);
-  if ((rc = run_poisson(c, 1))) return rc;
-  return correct(c, 3, "correct_vel_pr");
+  if ((rc = run_poisson(c, 1, ns, dev))) return rc;
+  return correct(c, 3, "correct_vel_pr", ns, dev);
+}
+
+int afid_substep(afid_ctx *c, int ns, double dt) {
+  if (!c || ns < 1 || ns > c->p.nsst || ns > 3 || dt <= 0) return AFID_ERR_ARG;
+  int rc = afid_set_substep(c, ns, dt);
+  if (rc) return rc;
+  if (c->fac_sub_dt[ns] != dt) {
+    if ((rc = upload_factors(c, c->al, dt, c->fac_sub[ns]))) return rc;
+    c->fac_sub_dt[ns] = dt;
+  }
+  return enqueue_substep(c, ns, false);
 }
 
 int afid_time_marcher(afid_ctx *c, double dt) {
@@ -1080,6 +1117,103 @@ int afid_step(afid_ctx *c, int ntime, int idtv, double dtin, double dtmin, doubl
   if (dt_used) *dt_used = dt;
   if (errorcode) *errorcode = err;
   return AFID_OK;
+}
+
+// ---- whole steps on the device ------------------------------------------------
+static int ensure_dev_step(afid_ctx *c) {
+  if (c->dstep) return AFID_OK;
+  const Geom &g = c->g;
+  HIPCHK(c, hipMalloc(&c->dstep, sizeof(DevStep)));
+  HIPCHK(c, hipMalloc(&c->fwork, sizeof(double) * SYS_N * 6 * (size_t)g.nx));
+  for (int q = 0; q < 3; q++) HIPCHK(c, hipMalloc(&c->fac_dev[q], sizeof(double) * SYS_N * FR_N * (size_t)g.ldx));
+  // natural-order rows (index kc-1) of the metric arrays the factorisation uses
+  const std::vector<double> *rows[7] = {&c->ac3ck, &c->am3ssk, &c->ap3ssk, &c->ac3sk,
+                                        &c->am3sk, &c->ap3sk, &c->ac3ssk};
+  std::vector<double> mc((size_t)7 * g.nx);
+  for (int q = 0; q < 7; q++)
+    for (int k = 0; k < g.nx; k++) mc[(size_t)q * g.nx + k] = (*rows[q])[k + 1];
+  HIPCHK(c, hipMalloc(&c->mc, sizeof(double) * mc.size()));
+  HIPCHK(c, hipMemcpy(c->mc, mc.data(), sizeof(double) * mc.size(), hipMemcpyHostToDevice));
+  c->bytes_held += sizeof(DevStep) + sizeof(double) * (SYS_N * 6 * (size_t)g.nx + 3 * SYS_N * FR_N * (size_t)g.ldx +
+                                                       mc.size());
+  StepConsts &k = c->kconst;
+  k.ren = c->ren; k.pec = c->pec; k.dy = c->dy; k.dz = c->dz; k.dyq = c->dyq; k.dzq = c->dzq;
+  k.tempbp = c->p.tempbp; k.temptp = c->p.temptp;
+  k.nsst = std::min(c->p.nsst, 3);
+  for (int ns = 1; ns <= 3; ns++) {
+    k.gam[ns - 1] = c->gam[ns];
+    k.rom[ns - 1] = c->rom[ns];
+    k.alm[ns - 1] = c->alm[ns];
+  }
+  return AFID_OK;
+}
+
+// One main.F90 step entirely on the stream: CalcMaxCFL, dt and substep
+// scalars, factorisations, the substeps, time += dt.
+static int enqueue_step_device(afid_ctx *c) {
+  int rc;
+  if ((rc = kerr(c, launch_reduce_cfl(reduce_args(c), c->st)))) return rc;
+  if ((rc = kerr(c, launch_step_begin(c->dstep, c->out, c->kconst, c->st)))) return rc;
+  FactorArgs fa;
+  fa.g = c->g;
+  fa.mc = c->mc;
+  for (int q = 0; q < 3; q++) fa.fac[q] = c->fac_dev[q];
+  fa.work = c->fwork;
+  fa.ds = c->dstep;
+  fa.k = c->kconst;
+  if ((rc = kerr(c, launch_factor(fa, c->st)))) return rc;
+  for (int ns = 1; ns <= c->kconst.nsst; ns++)
+    if ((rc = enqueue_substep(c, ns, true))) return rc;
+  return kerr(c, launch_step_end(c->dstep, c->st));
+}
+
+int afid_steps(afid_ctx *c, int ntime0, int nsteps, int idtv, double dt, double dtmin, double dtmax,
+               double limitcfl, double *dt_last, int *errorcode, int *steps_done) {
+  if (!c || nsteps < 0 || (!idtv && dt <= 0)) return AFID_ERR_ARG;
+  if (c->singular_mode) return AFID_NUM_SINGULAR;
+  if (!c->split) {   // the fused update swaps buffers per substep: step from the host
+    int err = 0, done = 0;
+    double dtu = dt;
+    for (int n = 0; n < nsteps && err == 0; n++, done++) {
+      int rc = afid_step(c, ntime0 + n, idtv, dt, dtmin, dtmax, limitcfl, &dtu, &err);
+      if (rc) return rc;
+    }
+    if (dt_last) *dt_last = dtu;
+    if (errorcode) *errorcode = err;
+    if (steps_done) *steps_done = done;
+    return AFID_OK;
+  }
+  int rc = ensure_dev_step(c);
+  if (rc) return rc;
+  DevStep h{};
+  h.time = c->time; h.ntime = ntime0; h.idtv = idtv; h.dtfix = dt; h.dtmin = dtmin; h.dtmax = dtmax;
+  h.limitcfl = limitcfl; h.dt = dt;
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  HIPCHK(c, hipMemcpy(c->dstep, &h, sizeof(DevStep), hipMemcpyHostToDevice));
+  if (!c->step_graph) {   // capture one step once; its kernels take no per-step host data
+    hipGraph_t graph = nullptr;
+    HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
+    rc = enqueue_step_device(c);
+    c->capturing = false;
+    hipError_t e = hipStreamEndCapture(c->st, &graph);
+    if (rc) {
+      if (graph) hipGraphDestroy(graph);
+      return rc;
+    }
+    HIPCHK(c, e);
+    e = hipGraphInstantiate(&c->step_graph, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    HIPCHK(c, e);
+  }
+  for (int n = 0; n < nsteps; n++) HIPCHK(c, hipGraphLaunch(c->step_graph, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  HIPCHK(c, hipMemcpy(&h, c->dstep, sizeof(DevStep), hipMemcpyDeviceToHost));
+  c->time = h.time;
+  if (dt_last) *dt_last = h.dt;
+  if (errorcode) *errorcode = h.errorcode;
+  if (steps_done) *steps_done = h.steps_done;
+  return h.pivot ? AFID_NUM_PIVOT : AFID_OK;
 }
 
 int afid_get_time(afid_ctx *c, double *t) {

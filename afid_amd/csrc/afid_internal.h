@@ -88,6 +88,24 @@ struct StepScalars {
   int use_ru;                    // ro != 0
 };
 
+// Step state kept on the device so that whole time steps run without a host
+// round trip (main.F90:167-200): dt selection from the CFL reduction, the
+// substep scalars, error codes and time.  `halted` is set at the end of a step
+// that raised an error code; every field-writing kernel of later steps then
+// returns at once, so a graph of many steps stops where main.F90 would.
+struct DevStep {
+  double dt, time, dtfix, dtmin, dtmax, limitcfl;
+  int ntime, idtv, errorcode, halted, pivot, steps_done;
+  StepScalars sc[3];   // scalars of substeps 1..3 of the current step
+};
+
+// Host-set constants the device step control needs to rebuild StepScalars.
+struct StepConsts {
+  double ren, pec, dy, dz, dyq, dzq, tempbp, temptp;
+  double gam[3], rom[3], alm[3];
+  int nsst;
+};
+
 struct FieldPtrs {
   const double *vx, *vy, *vz, *temp, *pr;
 };

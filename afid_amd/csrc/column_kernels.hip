@@ -580,7 +580,10 @@ __device__ __forceinline__ Plane load_plane(const double *__restrict__ vx, const
   return q;
 }
 
-template <typename OffT, bool USE_RU>
+// DEV: the substep scalars come from the device step state (a.ds) rather
+// than from the argument block; two instantiations keep each one's scalar
+// loads unconditional.
+template <typename OffT, bool USE_RU, bool DEV>
 __global__ void __launch_bounds__(STENCIL_WG)
     k_stencil_rhs(FusedArgs a, const double *__restrict__ vx, const double *__restrict__ vy,
                   const double *__restrict__ vz, const double *__restrict__ T, const double *__restrict__ pr,
@@ -598,6 +601,8 @@ __global__ void __launch_bounds__(STENCIL_WG)
   const int part = (int)(tile % nparts);
   const int64_t rest = tile / nparts;
   const int jc = (int)(rest % g.nym), ib = (int)(rest / g.nym);
+  if (DEV && a.ds->halted) return;
+  const StepScalars c = DEV ? a.ds->sc[a.ns - 1] : a.sc;
   const int s = part * STENCIL_WG + (int)threadIdx.x;
   for (int idx = threadIdx.x; idx < SC_N * STENCIL_WG; idx += STENCIL_WG) {
     const int r = idx / STENCIL_WG, sl = min(part * STENCIL_WG + idx - r * STENCIL_WG, ldx - 1);
@@ -617,7 +622,6 @@ __global__ void __launch_bounds__(STENCIL_WG)
   const OffT spb = (OffT)(k + 1 < nx ? pos_of_k(k + 1, L) : s) * sizeof(double);
   const bool inx = k >= 1 && k <= nx - 2;   // interior rows of the nx systems
   const bool iny = k <= nxm - 1;            // rows of the nxm systems
-  const StepScalars &c = a.sc;
   const int sl = (int)threadIdx.x;
   auto cfv = [&](int r) { return cf[r * STENCIL_WG + sl]; };
   const double udx3c = cfv(SC_UDX3C), udx3m = cfv(SC_UDX3M);
@@ -783,7 +787,7 @@ __global__ void __launch_bounds__(SOLVE_WG)
   extern __shared__ double rows[];   // [SYS_N][RW_N][ldx]
   const Geom g = a.g;
   const int64_t blk = xcd_block((g.ncol + SOLVE_COLS - 1) / SOLVE_COLS);
-  if (blk < 0) return;
+  if (blk < 0 || (a.ds && a.ds->halted)) return;
   const int L = g.L, ldx = g.ldx, nx = g.nx, nxm = g.nxm;
   // the factor rows are the same for every column: stage them once per block
   for (int idx = threadIdx.x; idx < SYS_N * RW_N * ldx; idx += SOLVE_WG) {
@@ -1012,7 +1016,7 @@ __global__ void __launch_bounds__(CORRECT_MAX_WG) k_correct(CorrectArgs a, int w
   const Geom g = a.g;
   const int nstrip = (g.nym + STRIP_W - 1) / STRIP_W;
   const int64_t row = xcd_block((int64_t)nstrip * g.nzm);
-  if (row < 0) return;
+  if (row < 0 || (a.ds && a.ds->halted)) return;
   const int strip = (int)(row / g.nzm), i = (int)(row - (int64_t)strip * g.nzm);
   const int ldx = g.ldx;
   for (int idx = threadIdx.x; idx < 4 * ldx; idx += blockDim.x) {
@@ -1023,7 +1027,7 @@ __global__ void __launch_bounds__(CORRECT_MAX_WG) k_correct(CorrectArgs a, int w
   __syncthreads();
   const int im = wrapm(i - 1, g.nzm), ip = wrapp(i + 1, g.nzm);
   const double *__restrict__ dph = a.dph;
-  const StepScalars &c = a.sc;
+  const StepScalars c = a.ds ? a.ds->sc[a.ns - 1] : a.sc;
   const int j0 = strip * STRIP_W, j1 = min(j0 + STRIP_W, g.nym);
   for (int j = j0; j < j1; j++) {
     const int jm = wrapm(j - 1, g.nym), jp = wrapp(j + 1, g.nym);
@@ -1088,7 +1092,10 @@ hipError_t column_configure_lds(size_t bytes) {
   AFID_LM_LIST(SETB)
 #undef SETB
 #define SETC(O, R)                                                                                       \
-  e = hipFuncSetAttribute((const void *)k_stencil_rhs<O, R>, hipFuncAttributeMaxDynamicSharedMemorySize,  \
+  e = hipFuncSetAttribute((const void *)k_stencil_rhs<O, R, true>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                          (int)bytes);                                                                   \
+  if (e != hipSuccess) return e;                                                                         \
+  e = hipFuncSetAttribute((const void *)k_stencil_rhs<O, R, false>, hipFuncAttributeMaxDynamicSharedMemorySize, \
                           (int)bytes);                                                                   \
   if (e != hipSuccess) return e;
   SETC(uint32_t, true) SETC(uint32_t, false) SETC(uint64_t, true) SETC(uint64_t, false)
@@ -1134,16 +1141,18 @@ hipError_t launch_stencil_rhs(const FusedArgs &a, hipStream_t st) {
   const dim3 grid((unsigned)(((ntile + 7) / 8) * 8));
   const size_t shm = stencil_lds(a.g);
   const bool small = (uint64_t)n * sizeof(double) < (1ull << 32);
-  const bool ru = a.sc.use_ru != 0;
-#define SR(O, R)                                                                                         \
-  if (std::is_same<O, uint32_t>::value == small && R == ru) {                                          \
-    hipLaunchKernelGGL((k_stencil_rhs<O, R>), grid, dim3(STENCIL_WG), shm, st, a, a.f.vx, a.f.vy, a.f.vz, \
+  const bool ru = a.sc.use_ru != 0, dev = a.ds != nullptr;
+#define SR1(O, R, D)                                                                                     \
+  if (std::is_same<O, uint32_t>::value == small && R == ru && D == dev) {                              \
+    hipLaunchKernelGGL((k_stencil_rhs<O, R, D>), grid, dim3(STENCIL_WG), shm, st, a, a.f.vx, a.f.vy, a.f.vz, \
                        a.f.temp, a.f.pr, a.rux, a.ruy, a.ruz, a.rut, a.bx, a.by, a.bz, a.bt, a.stale, a.kc, \
                        a.fac);                                                                         \
     return hipGetLastError();                                                                          \
   }
+#define SR(O, R) SR1(O, R, true) SR1(O, R, false)
   SR(uint32_t, true) SR(uint32_t, false) SR(uint64_t, true) SR(uint64_t, false)
 #undef SR
+#undef SR1
   return hipErrorInvalidValue;
 }
 

@@ -177,9 +177,11 @@ __global__ void __launch_bounds__(256) k_yfft_forward(PoissonArgs a) {
   const int ntile = g.ldx / TK;
   const int64_t tile = fft_tile((int64_t)ntile * g.nzm);
   if (tile < 0) return;
+  if (a.ds && a.ds->halted) return;
+  const StepScalars sc = a.ds ? a.ds->sc[a.ns - 1] : a.sc;
   const int s0 = (int)(tile % ntile) * TK, i = (int)(tile / ntile);
   const bool hsw = (H & 7) == 0;
-  const double usdtal = 1.0 / (a.sc.dt * a.sc.al);
+  const double usdtal = 1.0 / (sc.dt * sc.al);
   {
     const int q = threadIdx.x % TK, jr = threadIdx.x / TK;   // 32 rows per pass
     const int s = s0 + q, k = k_of_pos(s, g.L);
@@ -215,7 +217,7 @@ __global__ void __launch_bounds__(256) k_yfft_forward(PoissonArgs a) {
         if (j >= N) continue;
         double x = z0[b];
         if (a.from_velocity)
-          x = ((z1[b] - z0[b]) * a.sc.dz + (y1[b] - y0[b]) * a.sc.dy + (x1[b] - x0[b]) * u3) * usdtal;
+          x = ((z1[b] - z0[b]) * sc.dz + (y1[b] - y0[b]) * sc.dy + (x1[b] - x0[b]) * u3) * usdtal;
         const double v = live ? x : 0.0;
         double2 *e = &lds[q * H + swz(rv[b], hsw)];
         if (!py.half) *e = make_double2(v, 0.0);
@@ -266,7 +268,7 @@ __global__ void __launch_bounds__(256) k_zfft(PoissonArgs a, double sign) {
   const int N = a.pz.n;
   const int ntile = (a.nymh + TM - 1) / TM;
   const int64_t tile = fft_tile((int64_t)ntile * a.g.nxm);
-  if (tile < 0) return;
+  if (tile < 0 || (a.ds && a.ds->halted)) return;
   const int m0 = (int)(tile % ntile) * TM, k = (int)(tile / ntile);
   const bool zsw = (N & 7) == 0;
   double2 *S = a.S + (int64_t)k * a.g.nzm * a.ldm;
@@ -311,7 +313,7 @@ constexpr int XB = 8;
 __global__ void __launch_bounds__(256) k_xsolve_pressure(PoissonArgs a) {
   const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int nymh = a.nymh, nzm = a.g.nzm, nxm = a.g.nxm;
-  if (id >= (int64_t)nymh * nzm) return;
+  if (id >= (int64_t)nymh * nzm || (a.ds && a.ds->halted)) return;
   const int n = (int)(id / nymh), m = (int)(id - (int64_t)n * nymh);
   const int64_t stride = (int64_t)nzm * a.ldm;
   double2 *S = a.S + (int64_t)n * a.ldm + m;
@@ -379,6 +381,7 @@ __global__ void __launch_bounds__(256) k_yfft_inverse(PoissonArgs a) {
   const int ntile = g.ldx / TK;
   const int64_t tile = fft_tile((int64_t)ntile * g.nzm);
   if (tile < 0) return;
+  if (a.ds && a.ds->halted) return;
   const int s0 = (int)(tile % ntile) * TK, i = (int)(tile / ntile);
   const bool hsw = (H & 7) == 0;
   const int nymh = a.nymh;

@@ -14,6 +14,8 @@ struct FusedArgs {
   const double *fac;                 // SYS_N*FR_N interleaved factor rows
   double *bx, *by, *bz, *bt;         // right-hand sides b + A q_old of the split update
   StepScalars sc;
+  const DevStep *ds;                 // non-null: scalars of substep ns come from the device
+  int ns;
 };
 
 struct RoutineArgs {
@@ -42,6 +44,8 @@ struct CorrectArgs {
   double *vx, *vy, *vz, *pr;
   const double *kc;
   StepScalars sc;
+  const DevStep *ds;
+  int ns;
 };
 
 // One complex FFT length with its radix stages, twiddles exp(-2 pi i t/n)
@@ -79,6 +83,8 @@ struct PoissonArgs {
   RealPlan py;
   FFTPlan pz;
   int from_velocity;     // 1: y-FFT input is div(u)/(dt al) computed on the fly; 0: dph
+  const DevStep *ds;
+  int ns;
 };
 
 hipError_t launch_fused_update(const FusedArgs &a, hipStream_t st);
@@ -126,5 +132,20 @@ hipError_t launch_scatter_out(const double *src, double *stage, const Geom &g, i
 hipError_t launch_init_noise(double *temp, double *zero_fields[4], const double *xc_nat, const Geom &g,
                              uint64_t seed, double amp, double tbp, double ttp, hipStream_t st);
 hipError_t launch_fill(double *p, double v, int64_t n, hipStream_t st);
+
+// Device step control (DevStep): dt selection and substep scalars from the CFL
+// reduction result, the wall-normal factorisations of the step's substeps, and
+// the end-of-step bookkeeping.
+struct FactorArgs {
+  Geom g;
+  const double *mc;     // natural-order rows (index kc-1): ac3ck, am3ssk, ap3ssk, ac3sk, am3sk, ap3sk, ac3ssk
+  double *fac[3];       // factor rows per substep (SYS_N*FR_N*ldx each)
+  double *work;         // scratch, SYS_N * 6 * nx doubles
+  DevStep *ds;
+  StepConsts k;
+};
+hipError_t launch_step_begin(DevStep *ds, const double *cfl_out, StepConsts k, hipStream_t st);
+hipError_t launch_factor(const FactorArgs &a, hipStream_t st);
+hipError_t launch_step_end(DevStep *ds, hipStream_t st);
 
 }  // namespace afid

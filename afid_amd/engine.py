@@ -73,6 +73,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "afid_substep": ([vp, i, d], i),
         "afid_time_marcher": ([vp, d], i),
         "afid_step": ([vp, i, i, d, d, d, d, pd, ctypes.POINTER(i)], i),
+        "afid_steps": ([vp, i, i, i, d, d, d, d, pd, ctypes.POINTER(i), ctypes.POINTER(i)], i),
         "afid_get_time": ([vp, pd], i),
         "afid_set_time": ([vp, d], i),
         "afid_get_stale": ([vp, pd], i),
@@ -296,6 +297,16 @@ class Cell:
         self._check(self.L.afid_step(self.h, ntime, idtv, dt, dtmin, dtmax, limitcfl, ctypes.byref(used),
                                      ctypes.byref(err)), "step")
         return err.value, used.value
+
+    def steps(self, ntime0: int, nsteps: int, idtv: int, dt: float, dtmin: float = 1e-8, dtmax: float = 1e-2,
+              limitcfl: float = 1.2):
+        """nsteps main.F90 steps on the device (one captured graph per step);
+        returns (errorcode, dt of the last step run, steps run)."""
+        used = ctypes.c_double()
+        err, done = ctypes.c_int(), ctypes.c_int()
+        self._check(self.L.afid_steps(self.h, ntime0, nsteps, idtv, dt, dtmin, dtmax, limitcfl, ctypes.byref(used),
+                                      ctypes.byref(err), ctypes.byref(done)), "steps")
+        return err.value, used.value, done.value
 
     @property
     def time(self) -> float:

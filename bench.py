@@ -4,7 +4,8 @@ Workload (default): production single cell, nxm=384 (clipped-Chebyshev
 stretching istr3=6, str3=6), ny=nz=768 periodic, ylen=zlen=1, Ra=1e8, Pr=1,
 RK3, CFL-limited dt (cflmax 1.2, dtmin 1e-8, dtmax 1e-2), seeded conduction
 profile plus +-1e-3 noise, FP64.  One step = one main.F90 time step: CalcMaxCFL,
-dt selection, three fused substeps.  Inputs are resident in HBM before timing.
+dt selection, three substeps.  The timed steps run on the device from one
+captured graph per step (afid_steps); inputs are resident in HBM before timing.
 
 Prints one JSON line (metric Mcell-updates/s = nxm*nym*nzm*steps/s/1e6) with a
 roofline block for the dominant kernel (HIP events on the engine's stream) and a
@@ -140,17 +141,20 @@ def main():
     cell.init_linear_noise(cfg["seed"], 1e-3)
     cells = p["nxm"] * p["nym"] * p["nzm"]
     ctl = dict(idtv=1, dt=1e-3, dtmin=1e-8, dtmax=1e-2, limitcfl=1.2)
+    args_ctl = (ctl["idtv"], ctl["dt"], ctl["dtmin"], ctl["dtmax"], ctl["limitcfl"])
+    # whole steps on the device: CFL, dt, factorisations and the three substeps
+    # replayed from one captured graph per step; host sync only around the loop
     ntime = 0
-    for _ in range(args.warmup):
-        cell.step(ntime, ctl["idtv"], ctl["dt"], ctl["dtmin"], ctl["dtmax"], ctl["limitcfl"])
-        ntime += 1
+    err, _, done = cell.steps(ntime, args.warmup, *args_ctl)
+    ntime += done
     cell.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        err, dt = cell.step(ntime, ctl["idtv"], ctl["dt"], ctl["dtmin"], ctl["dtmax"], ctl["limitcfl"])
-        ntime += 1
+    err, _, done = cell.steps(ntime, args.steps, *args_ctl)
     cell.synchronize()
     elapsed = time.perf_counter() - t0
+    ntime += done
+    if err or done != args.steps:
+        raise RuntimeError(f"step control stopped early: errorcode {err} after {done} steps")
     ms_per_step = elapsed / args.steps * 1e3
     value = cells * args.steps / elapsed / 1e6
     divmax = cell.CheckDivergence()
@@ -158,8 +162,8 @@ def main():
     # per-kernel HIP-event timing on the engine stream (separate short pass)
     cell.profile(True)
     cell.profile_reset()
-    for _ in range(args.profile_steps):
-        cell.step(ntime, ctl["idtv"], ctl["dt"], ctl["dtmin"], ctl["dtmax"], ctl["limitcfl"])
+    for _ in range(args.profile_steps):   # host-driven steps: each kernel bracketed by HIP events
+        cell.step(ntime, *args_ctl)
         ntime += 1
     prof = cell.profile_read()
     cell.profile(False)

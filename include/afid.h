@@ -134,6 +134,16 @@ int afid_time_marcher(afid_ctx *ctx, double dt);
  * time += dt.  *errorcode receives 0, 165 (CFL too high) or 166 (dt<dtmin). */
 int afid_step(afid_ctx *ctx, int ntime, int idtv, double dt, double dtmin, double dtmax,
               double limitcfl, double *dt_used, int *errorcode);
+/* main.F90:167-200 for nsteps consecutive steps (ntime = ntime0, ntime0+1, ...)
+ * without host round trips: CalcMaxCFL, dt selection, the substeps with their
+ * factorisations and time += dt all run on the device from one captured graph
+ * per step (the first call captures it).  A step that raises errorcode 165 or
+ * 166 still completes, as in main.F90, and the remaining steps are skipped on
+ * the device.  Returns the dt of the last step run, its errorcode and the
+ * number of steps run.  AFID_NUM_PIVOT if a factorisation would need a row
+ * interchange. */
+int afid_steps(afid_ctx *ctx, int ntime0, int nsteps, int idtv, double dt, double dtmin, double dtmax,
+               double limitcfl, double *dt_last, int *errorcode, int *steps_done);
 int afid_get_time(afid_ctx *ctx, double *time);
 int afid_set_time(afid_ctx *ctx, double time);      /* time of a continuation (ReadFlowField.F90:46) */
 /* The stale row rhs(1,j,i) the x-solves read (SolveImpEqnUpdate_X.F90:44):

@@ -253,3 +253,49 @@ def test_fused_and_split_update_agree(afid, c, monkeypatch):
     for nm in ("vx", "vy", "vz", "temp"):
         assert rel_l2(out["split"][nm], out["fused"][nm]) < 1e-13, nm
         assert rel_l2(out["split"][nm], o.interior(nm)) < 1e-11, nm
+
+
+@pytest.mark.parametrize("c", SMALL_CASES, ids=case_id)
+def test_device_steps_match_host_steps(afid, c):
+    """afid_steps (dt, factorisations and substeps on the device, one captured
+    graph per step) against afid_step driven from the host, and the oracle."""
+    o, host = make_pair(afid, c)
+    dev = afid.Cell(afid.CellParams(nsst=3, **dict(c, alx3=c.get("alx3", 1.0))))
+    for x in (o, host, dev):
+        x.init_linear_noise(4, 1e-3)
+    dts = []
+    for n in range(8):
+        err, dt = host.step(n, 1, 2e-3, 1e-4, 5e-3, 1.2)
+        o.step(n, 1, 2e-3, 1e-4, 5e-3, 1.2)
+        dts.append(dt)
+    e1, dt1, k1 = dev.steps(0, 5, 1, 2e-3, 1e-4, 5e-3, 1.2)   # two calls: the graph is replayed
+    e2, dt2, k2 = dev.steps(5, 3, 1, 2e-3, 1e-4, 5e-3, 1.2)
+    assert (e1, e2, k1, k2) == (0, 0, 5, 3)
+    assert dt2 == pytest.approx(dts[-1], rel=1e-12)
+    assert dev.time == pytest.approx(host.time, rel=1e-12)
+    for nm in ("vx", "vy", "vz", "temp"):
+        assert rel_l2(dev.download(nm), host.download(nm)) < 1e-13, nm
+        assert rel_l2(dev.download(nm), o.interior(nm)) < 1e-11, nm
+    nxm = o.nxm
+    assert rel_l2(demean(dev.download("pr")[:, :, :nxm]), demean(o.interior("pr")[:, :, :nxm])) < 1e-11
+    host.close()
+    dev.close()
+
+
+def test_device_steps_stop_after_error_code(afid):
+    """dtmax < dtmin: the third step (ntime = 2, CFL dt clipped to dtmax) raises
+    166; main.F90 finishes that step and quits, and so does afid_steps."""
+    c = SMALL_CASES[0]
+    host = afid.Cell(afid.CellParams(**c))
+    dev = afid.Cell(afid.CellParams(**c))
+    host.init_linear_noise(2, 1e-3)
+    dev.init_linear_noise(2, 1e-3)
+    for n in range(3):
+        err, dt = host.step(n, 1, 1e-3, 1e-3, 5e-4, 1.2)
+    assert err == 166
+    e, dtl, done = dev.steps(0, 6, 1, 1e-3, 1e-3, 5e-4, 1.2)
+    assert (e, done) == (166, 3) and dtl == pytest.approx(dt, rel=1e-12)
+    for nm in ("vx", "vy", "vz", "temp", "pr"):
+        assert rel_l2(dev.download(nm), host.download(nm)) < 1e-13, nm
+    host.close()
+    dev.close()
