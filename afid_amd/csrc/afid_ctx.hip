@@ -58,6 +58,14 @@ struct afid_ctx {
   StepConsts kconst{};
   hipGraphExec_t step_graph = nullptr;
   bool capturing = false;
+  // statistics (StatRoutines.F90): device accumulators [STATS_N][nxm], their
+  // block partials, the dissipation profiles (host) and the sample count
+  double *stat_acc = nullptr, *stat_partial = nullptr;
+  std::vector<double> disste_acc, dissth_acc;
+  int nstatsamples = 0;
+  int *d_kslab = nullptr;
+  double *slab_buf = nullptr;
+  int slab_n = 0;
   double *stale = nullptr;     // rhs(1,j,i)
   double *kc = nullptr;
   double *fac_x = nullptr;     // per-routine factors (SYS_N*FR_N*ldx), current coefficients
@@ -603,6 +611,10 @@ int afid_destroy(afid_ctx *c) {
   hipFree(c->dstep);
   hipFree(c->mc);
   hipFree(c->fwork);
+  hipFree(c->stat_acc);
+  hipFree(c->stat_partial);
+  hipFree(c->d_kslab);
+  hipFree(c->slab_buf);
   for (int q = 0; q < 4; q++) hipFree(c->fac_sub[q]);
   void *ps[] = {c->stale, c->kc, c->fac_x, c->S, c->cp, c->nat, c->partial, c->out, c->stage,
                 c->dpy.tw, c->dpy.rev, c->dpy.post, c->dpz.tw, c->dpz.rev, c->dpz.post};
@@ -1296,6 +1308,77 @@ int afid_calc_dissipation_nu(afid_ctx *c, double *nu_mu, double *nu_th, double *
   *nu_th = v[1] * volt;
   if (disste) std::copy(v.begin() + 2, v.begin() + 2 + c->g.nxm, disste);
   if (dissth) std::copy(v.begin() + 2 + c->g.nxm, v.end(), dissth);
+  // the profiles also accumulate into the statistics (CalcDissipationNu.F90:101-104)
+  c->disste_acc.resize(c->g.nxm, 0.0);
+  c->dissth_acc.resize(c->g.nxm, 0.0);
+  for (int k = 0; k < c->g.nxm; k++) {
+    c->disste_acc[k] += v[2 + k];
+    c->dissth_acc[k] += v[2 + c->g.nxm + k];
+  }
+  return AFID_OK;
+}
+
+// ---- statistics and slabs -------------------------------------------------------
+int afid_calc_stats(afid_ctx *c) {
+  if (!c) return AFID_ERR_ARG;
+  if (!c->stat_acc) {
+    HIPCHK(c, hipMalloc(&c->stat_acc, sizeof(double) * STATS_N * c->g.nxm));
+    HIPCHK(c, hipMemsetAsync(c->stat_acc, 0, sizeof(double) * STATS_N * c->g.nxm, c->st));
+    HIPCHK(c, hipMalloc(&c->stat_partial, sizeof(double) * stats_partial_doubles(c->g)));
+  }
+  ReduceArgs a = reduce_args(c);
+  a.partial = c->stat_partial;
+  int rc = kerr(c, launch_stats(a, c->stat_acc, c->st));
+  if (rc) return rc;
+  c->nstatsamples += 1;
+  return AFID_OK;
+}
+
+int afid_get_stats(afid_ctx *c, double *out, int *nsamples) {
+  if (!c || !out) return AFID_ERR_ARG;
+  const int nxm = c->g.nxm;
+  std::fill(out, out + (STATS_N + 2) * nxm, 0.0);
+  if (c->stat_acc) {
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    HIPCHK(c, hipMemcpy(out, c->stat_acc, sizeof(double) * STATS_N * nxm, hipMemcpyDeviceToHost));
+  }
+  for (int k = 0; k < nxm && k < (int)c->disste_acc.size(); k++) {
+    out[STATS_N * nxm + k] = c->disste_acc[k];
+    out[(STATS_N + 1) * nxm + k] = c->dissth_acc[k];
+  }
+  if (nsamples) *nsamples = c->nstatsamples;
+  return AFID_OK;
+}
+
+int afid_reset_stats(afid_ctx *c) {
+  if (!c) return AFID_ERR_ARG;
+  if (c->stat_acc) HIPCHK(c, hipMemsetAsync(c->stat_acc, 0, sizeof(double) * STATS_N * c->g.nxm, c->st));
+  std::fill(c->disste_acc.begin(), c->disste_acc.end(), 0.0);
+  std::fill(c->dissth_acc.begin(), c->dissth_acc.end(), 0.0);
+  c->nstatsamples = 0;
+  return AFID_OK;
+}
+
+int afid_get_slabs(afid_ctx *c, const int *kslab, int nslab, double *out) {
+  if (!c || !kslab || !out || nslab < 1 || nslab > 64) return AFID_ERR_ARG;
+  for (int m = 0; m < nslab; m++)
+    if (kslab[m] < 1 || kslab[m] > c->g.nxm) return AFID_ERR_ARG;
+  if (nslab > c->slab_n) {
+    hipFree(c->d_kslab);
+    hipFree(c->slab_buf);
+    c->d_kslab = nullptr;
+    c->slab_buf = nullptr;
+    c->slab_n = 0;
+    HIPCHK(c, hipMalloc(&c->d_kslab, sizeof(int) * nslab));
+    HIPCHK(c, hipMalloc(&c->slab_buf, sizeof(double) * 4 * (size_t)nslab * c->g.ncol));
+    c->slab_n = nslab;
+  }
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  HIPCHK(c, hipMemcpy(c->d_kslab, kslab, sizeof(int) * nslab, hipMemcpyHostToDevice));
+  int rc = kerr(c, launch_slabs(fields(c), c->g, c->d_kslab, nslab, c->slab_buf, c->st));
+  if (rc) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  HIPCHK(c, hipMemcpy(out, c->slab_buf, sizeof(double) * 4 * (size_t)nslab * c->g.ncol, hipMemcpyDeviceToHost));
   return AFID_OK;
 }
 

@@ -123,6 +123,11 @@ hipError_t launch_reduce_global(const ReduceArgs &a, hipStream_t st);     // out
 hipError_t launch_reduce_platenu(const ReduceArgs &a, double del, double deln, hipStream_t st);  // out[0..1]
 hipError_t launch_reduce_dissipation(const ReduceArgs &a, hipStream_t st);  // out[0..1], out[2..] profiles
 int reduce_partial_blocks();
+constexpr int STATS_N = 9;   // vx, vy, vz, T means; vx^2, vy^2, vz^2, T^2; T*vx
+hipError_t launch_stats(const ReduceArgs &a, double *acc, hipStream_t st);   // acc[STATS_N][nxm] +=
+size_t stats_partial_doubles(const Geom &g);
+hipError_t launch_slabs(const FieldPtrs &f, const Geom &g, const int *kslab, int nslab, double *out,
+                        hipStream_t st);
 
 // Layout conversion host-array staging <-> interleaved device field.
 hipError_t launch_gather_in(const double *stage, double *dst, const Geom &g, int n1, int ld1, int ld2,

@@ -192,6 +192,68 @@ __global__ void k_red_diss(ReduceArgs a) {
   }
 }
 
+// CalcStats (StatRoutines.F90:22-44): plane sums over (j, i) of vx, vy, vz, T,
+// their squares and T*vx, per k.  Pass 1: block b sums its contiguous range of
+// columns into partial[b][q][s] (thread = stored position s, so the loads are
+// coalesced along the column); pass 2 adds the blocks' partials in a fixed
+// order and accumulates sum * (1/nzm)(1/nym) into acc[q][k].
+constexpr int SBLK = 256;
+__global__ void __launch_bounds__(256) k_stats_partial(ReduceArgs a) {
+  const Geom g = a.g;
+  const int64_t cpb = (g.ncol + SBLK - 1) / SBLK;
+  const int64_t c0 = blockIdx.x * cpb, c1 = min(c0 + cpb, g.ncol);
+  for (int s = threadIdx.x; s < g.ldx; s += blockDim.x) {
+    double q[STATS_N] = {0};
+    for (int64_t c = c0; c < c1; c++) {
+      const int64_t o = c * g.ldx + s;
+      const double vx = a.f.vx[o], vy = a.f.vy[o], vz = a.f.vz[o], T = a.f.temp[o];
+      q[0] += vx; q[1] += vy; q[2] += vz; q[3] += T;
+      q[4] += vx * vx; q[5] += vy * vy; q[6] += vz * vz; q[7] += T * T; q[8] += T * vx;
+    }
+    for (int r = 0; r < STATS_N; r++) a.partial[((int64_t)blockIdx.x * STATS_N + r) * g.ldx + s] = q[r];
+  }
+}
+__global__ void k_stats_finish(ReduceArgs a, double *acc) {
+  const Geom g = a.g;
+  const double w = (1.0 / g.nzm) * (1.0 / g.nym);
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < STATS_N * g.nxm; idx += gridDim.x * blockDim.x) {
+    const int r = idx / g.nxm, k = idx - r * g.nxm, s = pos_of_k(k, g.L);
+    double v = 0.0;
+    for (int b = 0; b < SBLK; b++) v += a.partial[((int64_t)b * STATS_N + r) * g.ldx + s];
+    acc[r * g.nxm + k] += v * w;
+  }
+}
+hipError_t launch_stats(const ReduceArgs &a, double *acc, hipStream_t st) {
+  hipLaunchKernelGGL(k_stats_partial, dim3(SBLK), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_stats_finish, dim3(16), dim3(256), 0, st, a, acc);
+  return hipGetLastError();
+}
+size_t stats_partial_doubles(const Geom &g) { return (size_t)SBLK * STATS_N * g.ldx; }
+
+// SlabDumper (SlabDumpRoutines.F90:20-31): planes k = kslab(m) of
+// (vx(k)+vx(k+1))/2, vy, vz, temp for the nslab levels, as
+// out[((m*4 + q) * nzm + i) * nym + j] (Fortran (j, i) order per plane).
+__global__ void k_slabs(FieldPtrs f, Geom g, const int *kslab, int nslab, double *out) {
+  const int64_t n = (int64_t)nslab * g.ncol;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += (int64_t)gridDim.x * blockDim.x) {
+    const int m = (int)(idx / g.ncol);
+    const int64_t c = idx - (int64_t)m * g.ncol;   // c = i * nym + j
+    const int k = kslab[m] - 1;
+    const int64_t o = c * g.ldx;
+    const int s = pos_of_k(k, g.L), sp = pos_of_k(min(k + 1, g.nx - 1), g.L);
+    double *dst = out + (int64_t)m * 4 * g.ncol + c;
+    dst[0] = (f.vx[o + s] + f.vx[o + sp]) * 0.5;
+    dst[g.ncol] = f.vy[o + s];
+    dst[2 * g.ncol] = f.vz[o + s];
+    dst[3 * g.ncol] = f.temp[o + s];
+  }
+}
+hipError_t launch_slabs(const FieldPtrs &f, const Geom &g, const int *kslab, int nslab, double *out,
+                        hipStream_t st) {
+  hipLaunchKernelGGL(k_slabs, dim3(4096), dim3(256), 0, st, f, g, kslab, nslab, out);
+  return hipGetLastError();
+}
+
 static hipError_t finalize(const ReduceArgs &a, int nq, int maxmask, int minmask, hipStream_t st) {
   hipLaunchKernelGGL(k_finalize, dim3(nq), dim3(256), 0, st, a.partial, a.out, nq, (const int *)nullptr,
                      maxmask, minmask);

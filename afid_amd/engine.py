@@ -77,6 +77,10 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "afid_get_time": ([vp, pd], i),
         "afid_set_time": ([vp, d], i),
         "afid_get_stale": ([vp, pd], i),
+        "afid_calc_stats": ([vp], i),
+        "afid_get_stats": ([vp, pd, ctypes.POINTER(i)], i),
+        "afid_reset_stats": ([vp], i),
+        "afid_get_slabs": ([vp, ctypes.POINTER(i), i, pd], i),
         "afid_set_stale": ([vp, pd], i),
         "afid_calc_max_cfl": ([vp, pd], i),
         "afid_check_divergence": ([vp, pd], i),
@@ -116,7 +120,10 @@ def load_io_library(path: str = IO_LIB_PATH) -> ctypes.CDLL:
         raise AfidError(f"{path} not found: build it with afid_amd.build.build()")
     L = ctypes.CDLL(path)
     vp, i, pd, cp = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_char_p
+    pi = ctypes.POINTER(ctypes.c_int)
     sig = {"afid_write_continuation": ([vp, cp], i), "afid_read_continuation": ([vp, cp], i),
+           "afid_write_stats": ([vp, cp, i, i], i), "afid_slab_levels": ([pd, i, pd, pi], i),
+           "afid_write_slabs": ([vp, cp, pi, ctypes.c_double, ctypes.c_double], i),
            "afid_h5_write_field": ([cp, cp, pd, i, i, i], i), "afid_h5_read_field": ([cp, cp, pd, i, i, i], i)}
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -132,6 +139,21 @@ def exported_symbols(header: str = "afid.h") -> list[str]:
     import re
     names = re.findall(r"^(?:int|const char \*)\s*(afid_\w+)\s*\(", open(hdr).read(), flags=re.M)
     return names
+
+
+STAT_NAMES = ("vx_mean", "vy_mean", "vz_mean", "temp_mean", "vx_rms", "vy_rms", "vz_rms", "temp_rms",
+              "tempvx_mean", "disste", "dissth")
+
+
+def slab_levels(xm: np.ndarray, xslab: np.ndarray) -> np.ndarray:
+    """InitializeSlabDump: kslab(1..9) for xm(1..nxm) and xslab(2..9)."""
+    xm = np.ascontiguousarray(xm, dtype=np.float64)
+    xs = np.ascontiguousarray(xslab, dtype=np.float64)
+    out = (ctypes.c_int * 9)()
+    rc = load_io_library().afid_slab_levels(_dp(xm), len(xm), _dp(xs), out)
+    if rc:
+        raise AfidError(f"slab_levels: {rc}")
+    return np.array(out[:], dtype=np.int32)
 
 
 def h5_write_field(path: str, dset: str, a: np.ndarray) -> None:
@@ -317,6 +339,35 @@ class Cell:
     @time.setter
     def time(self, t: float):
         self._check(self.L.afid_set_time(self.h, t), "set_time")
+
+    # ---- statistics and slabs (StatRoutines.F90 / SlabDumpRoutines.F90) ---------
+    def CalcStats(self):
+        self._check(self.L.afid_calc_stats(self.h), "CalcStats")
+
+    def stats(self):
+        """Accumulated statistics {name: array(nxm)} and the number of samples."""
+        out = np.zeros(11 * self.nxm)
+        n = ctypes.c_int()
+        self._check(self.L.afid_get_stats(self.h, _dp(out), ctypes.byref(n)), "get_stats")
+        return {nm: out[q * self.nxm:(q + 1) * self.nxm] for q, nm in enumerate(STAT_NAMES)}, n.value
+
+    def reset_stats(self):
+        self._check(self.L.afid_reset_stats(self.h), "reset_stats")
+
+    def WriteStats(self, directory: str = ".", readstats: bool = False, disscal: bool = True):
+        self._check(load_io_library().afid_write_stats(self.h, directory.encode(), int(readstats), int(disscal)),
+                    "WriteStats")
+
+    def slabs(self, kslab) -> np.ndarray:
+        """Planes [m, q, i, j] at levels kslab (1-based): q = (vx(k)+vx(k+1))/2, vy, vz, temp."""
+        ks = (ctypes.c_int * len(kslab))(*[int(k) for k in kslab])
+        out = np.zeros((len(kslab), 4, self.nzm, self.nym))
+        self._check(self.L.afid_get_slabs(self.h, ks, len(kslab), _dp(out)), "get_slabs")
+        return out
+
+    def SlabDumper(self, directory: str, kslab, time: float, tout: float):
+        ks = (ctypes.c_int * 9)(*[int(k) for k in kslab])
+        self._check(load_io_library().afid_write_slabs(self.h, directory.encode(), ks, time, tout), "SlabDumper")
 
     # ---- continuation files (WriteFlowField.F90 / ReadFlowField.F90) -------------
     def WriteFlowField(self, directory: str = "."):

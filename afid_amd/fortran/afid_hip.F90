@@ -255,6 +255,34 @@
           import :: c_int, c_ptr
           type(c_ptr), value :: ctx
         end function
+!       statistics and slabs (StatRoutines.F90, SlabDumpRoutines.F90)
+        integer(c_int) function afid_calc_stats(ctx)                    &
+     &      bind(C, name='afid_calc_stats')
+          import :: c_int, c_ptr
+          type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function afid_write_stats(ctx, dir, readstats,    &
+     &      disscal) bind(C, name='afid_write_stats')
+          import :: c_int, c_ptr, c_char
+          type(c_ptr), value :: ctx
+          character(kind=c_char), intent(in) :: dir(*)
+          integer(c_int), value :: readstats, disscal
+        end function
+        integer(c_int) function afid_slab_levels(xm, nxm, xslab8,        &
+     &      kslab9) bind(C, name='afid_slab_levels')
+          import :: c_int, c_double
+          real(c_double), intent(in) :: xm(*), xslab8(8)
+          integer(c_int), value :: nxm
+          integer(c_int), intent(out) :: kslab9(9)
+        end function
+        integer(c_int) function afid_write_slabs(ctx, dir, kslab9, time, &
+     &      tout) bind(C, name='afid_write_slabs')
+          import :: c_int, c_ptr, c_char, c_double
+          type(c_ptr), value :: ctx
+          character(kind=c_char), intent(in) :: dir(*)
+          integer(c_int), intent(in) :: kslab9(9)
+          real(c_double), value :: time, tout
+        end function
       end interface
 
       contains

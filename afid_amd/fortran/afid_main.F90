@@ -8,7 +8,9 @@
 !  error codes), TimeMarcher.F90:13-78 (substep sequence), the log files
 !  of CalcPlateNu.F90:49-53, GlobalQuantities.F90:81-97 and
 !  CalcDissipationNu.F90:114-116; ReadFlowField.F90:29-45,160-176 (nread)
-!  and QuitRoutine.F90:27-31 (continuation written at a normal exit).
+!  and QuitRoutine.F90:27-31 (statistics and continuation written at a
+!  normal exit); StatRoutines.F90:12-44 and SlabDumpRoutines.F90:12-70
+!  at the output times after tsta (main.F90:210-216), slabs into stst3/.
 !
 !  Usage: afid_run [bou.in] [--seed N] [--routines] [--out DIR]
 !                  [--parse-only] [--abi-check]
@@ -27,6 +29,8 @@
       integer :: nxm, nym, nzm, nsst, nread, ntst, ireset, istr3
       integer :: flagstat, flagbal, starea, inslws, inslwn, idtv
       integer :: stst3flag
+      logical :: dumpslabs = .false.
+      integer(c_int) :: kslab(9) = 2
       real(8) :: walltimemax, tout, tmax, alx3, str3, ylen, zlen
       real(8) :: ray, pra, dt, resid, limitcfl, tsta, dtmin, dtmax
       real(8) :: limitvel
@@ -62,6 +66,11 @@
         read(u,'(a4)') skip
         read(u,*) stst3flag
         close(u)
+        if (stst3flag /= 0) then
+          inquire(file='./stst3.in', exist=dumpslabs)
+          if (.not. dumpslabs)                                            &
+     &      write(6,*) 'stst3.in not found, turning off slab dump'
+        end if
       end subroutine ReadInputFile
 
 !     One full time step of the reference's TimeMarcher: per-routine
@@ -122,6 +131,11 @@
      &    position='append')
         write(u,'(4(1x,e14.6))') time, nub, nut
         close(u)
+        if (time > tsta .and. flagstat /= 0)                              &
+     &    call afid_check(afid_calc_stats(ctx), 'CalcStats')
+        if (time > tsta .and. dumpslabs)                                  &
+     &    call afid_check(afid_write_slabs(ctx, 'stst3'//c_null_char,     &
+     &      kslab, time, tout), 'SlabDumper')
         if (time > tsta .and. flagstat /= 0 .and. flagbal /= 0) then
           call afid_check(afid_calc_dissipation_nu(ctx, numu, nuth,       &
      &      c_null_ptr, c_null_ptr), 'CalcDissipationNu')
@@ -131,6 +145,28 @@
           close(u)
         end if
       end subroutine WriteDiagnostics
+
+!     InitializeSlabDump (SlabDumpRoutines.F90:40-70): levels of the
+!     xslab(2:9) read from stst3.in, written to stst3locs.out.
+      subroutine InitializeSlabDump
+        character(len=4) :: skip
+        real(8) :: xslab(8)
+        real(8), allocatable :: xm(:)
+        integer :: u
+        open(newunit=u, file='stst3.in', status='old', action='read')
+        read(u,'(a4)') skip
+        read(u,*) xslab
+        close(u)
+        allocate(xm(nxm))
+        call afid_check(afid_get_metric(ctx, 'xm'//c_null_char, xm, nxm),  &
+     &    'InitializeSlabDump')
+        call afid_check(afid_slab_levels(xm, nxm, xslab, kslab),          &
+     &    'InitializeSlabDump')
+        deallocate(xm)
+        open(newunit=u, file='stst3locs.out', status='unknown')
+        write(u,*) kslab
+        close(u)
+      end subroutine InitializeSlabDump
 
       subroutine WriteFields(dir)
         character(len=*), intent(in) :: dir
@@ -270,6 +306,7 @@
      &    'CreateInitialConditions')
         time = 0.d0
       end if
+      if (dumpslabs) call InitializeSlabDump
       call afid_check(afid_check_divergence(ctx, dmax), 'CheckDivergence')
       write(6,*) ' Initial maximum divergence: ', dmax
 
@@ -339,6 +376,11 @@
       write(6,'(a,i0,a,es23.16)') 'errorcode ', errorcode, ' time ', time
       if (errorcode == 333 .or. errorcode == 334 .or. errorcode == 555)   &
      &  then
+        if (flagstat /= 0) then
+          call afid_check(afid_write_stats(ctx, '.'//c_null_char,         &
+     &      merge(1, 0, starea /= 0), merge(1, 0, flagbal /= 0)),         &
+     &      'WriteStats')
+        end if
         call afid_check(afid_set_time(ctx, time), 'WriteFlowField')
         call afid_check(afid_write_continuation(ctx, '.'//c_null_char),  &
      &    'WriteFlowField')

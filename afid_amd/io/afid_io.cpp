@@ -1,4 +1,4 @@
-// afid_io.cpp -- continuation files (include/afid_io.h) over the public C ABI
+// afid_io.cpp -- continuation, statistics and slab files (include/afid_io.h) over the public C ABI
 // of libafid_hip and the HDF5 C library.  Host code only.
 //
 // Layout followed: WriteFlowField.F90:17-45 (file names, master scalars),
@@ -6,8 +6,10 @@
 // NATIVE_INTEGER), :388-477 (dataset 'var', Fortran dims (nx, nym, nzm), i.e.
 // C dims {nzm, nym, nx}), ReadFlowField.F90:29-45 (master read), :160-170.
 #include <hdf5.h>
+#include <sys/stat.h>
 
 #include <cmath>
+#include <cstdio>
 #include <string>
 #include <vector>
 
@@ -181,6 +183,92 @@ int afid_write_continuation(afid_ctx *ctx, const char *dir) {
   if (!rc) rc = write_dataset(f, "str3", H5T_NATIVE_DOUBLE, &str3, 1, one);
   if (H5Fclose(f) < 0 && !rc) rc = AFID_ERR_IO;
   return rc;
+}
+
+int afid_write_stats(afid_ctx *ctx, const char *dir, int readstats, int disscal) {
+  if (!ctx) return AFID_ERR_ARG;
+  H5Silence quiet;
+  Dims d;
+  int rc = dims_of(ctx, &d);
+  if (rc) return rc;
+  const int nxm = d.nxm;
+  std::vector<double> st((size_t)11 * nxm);
+  int nsamples = 0;
+  if ((rc = afid_get_stats(ctx, st.data(), &nsamples))) return rc;
+  const char *names[11] = {"vx_mean", "vy_mean", "vz_mean", "temp_mean", "vx_rms", "vy_rms",
+                           "vz_rms", "temp_rms", "tempvx_mean", "disste", "dissth"};
+  const int nq = disscal ? 11 : 9;
+  const std::string path = join(dir, "stafield_master.h5");
+  const hsize_t one[1] = {1}, dn[1] = {(hsize_t)nxm};
+  hid_t f = -1;
+  if (readstats) f = H5Fopen(path.c_str(), H5F_ACC_RDWR, H5P_DEFAULT);
+  if (f >= 0) {   // add what the file holds
+    int old = 0;
+    if ((rc = read_dataset(f, "averaging_time", H5T_NATIVE_INT, &old, 1, one))) {
+      H5Fclose(f);
+      return rc;
+    }
+    nsamples += old;
+    std::vector<double> prev(nxm);
+    for (int q = 0; q < nq; q++) {
+      if ((rc = read_dataset(f, names[q], H5T_NATIVE_DOUBLE, prev.data(), 1, dn))) {
+        H5Fclose(f);
+        return rc;
+      }
+      for (int k = 0; k < nxm; k++) st[(size_t)q * nxm + k] += prev[k];
+    }
+  } else {
+    f = H5Fcreate(path.c_str(), H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
+    if (f < 0) return AFID_ERR_IO;
+  }
+  for (int q = 0; q < nq && !rc; q++) rc = write_dataset(f, names[q], H5T_NATIVE_DOUBLE, &st[(size_t)q * nxm], 1, dn);
+  std::vector<double> xm(nxm);
+  double ray = 0.0, pra = 0.0;
+  if (!rc) rc = write_dataset(f, "averaging_time", H5T_NATIVE_INT, &nsamples, 1, one);
+  if (!rc) rc = afid_get_metric(ctx, "xm", xm.data(), nxm);
+  if (!rc) rc = write_dataset(f, "X_cordin", H5T_NATIVE_DOUBLE, xm.data(), 1, dn);
+  if (!rc) rc = afid_get_metric(ctx, "ray", &ray, 1);
+  if (!rc) rc = write_dataset(f, "Rayleigh Number", H5T_NATIVE_DOUBLE, &ray, 1, one);
+  if (!rc) rc = afid_get_metric(ctx, "pra", &pra, 1);
+  if (!rc) rc = write_dataset(f, "Prandtl Number", H5T_NATIVE_DOUBLE, &pra, 1, one);
+  if (H5Fclose(f) < 0 && !rc) rc = AFID_ERR_IO;
+  return rc;
+}
+
+int afid_slab_levels(const double *xm, int nxm, const double *xslab8, int *kslab9) {
+  if (!xm || !xslab8 || !kslab9 || nxm < 2) return AFID_ERR_ARG;
+  for (int m = 0; m < 9; m++) kslab9[m] = 2;
+  for (int k = 2; k <= nxm; k++)
+    for (int j = 2; j <= 9; j++)
+      if (xm[k - 1] > xslab8[j - 2] && xm[k - 2] < xslab8[j - 2]) kslab9[j - 1] = k;
+  return AFID_OK;
+}
+
+int afid_write_slabs(afid_ctx *ctx, const char *dir, const int *kslab9, double time, double tout) {
+  if (!ctx || !kslab9 || tout <= 0) return AFID_ERR_ARG;
+  H5Silence quiet;
+  Dims d;
+  int rc = dims_of(ctx, &d);
+  if (rc) return rc;
+  const size_t plane = (size_t)d.nym * d.nzm;
+  mkdir(dir, 0755);   // the reference expects ./stst3 to exist; an existing one is kept
+  std::vector<double> buf(9 * 4 * plane);
+  if ((rc = afid_get_slabs(ctx, kslab9, 9, buf.data()))) return rc;
+  const long itime = std::lround(time * (1.0 / tout));
+  const char *qn[4] = {"vx", "vy", "vz", "temp"};
+  const hsize_t dims[2] = {(hsize_t)d.nzm, (hsize_t)d.nym}, one[1] = {1};
+  for (int m = 0; m < 9; m++)
+    for (int q = 0; q < 4; q++) {
+      char name[64];
+      std::snprintf(name, sizeof name, "slab%d%s_%08ld.h5", m + 1, qn[q], itime);
+      hid_t f = H5Fcreate(join(dir, name).c_str(), H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
+      if (f < 0) return AFID_ERR_IO;
+      rc = write_dataset(f, "var", H5T_NATIVE_DOUBLE, &buf[(m * 4 + q) * plane], 2, dims);
+      if (!rc) rc = write_dataset(f, "time", H5T_NATIVE_DOUBLE, &time, 1, one);
+      if (H5Fclose(f) < 0 && !rc) rc = AFID_ERR_IO;
+      if (rc) return rc;
+    }
+  return AFID_OK;
 }
 
 int afid_read_continuation(afid_ctx *ctx, const char *dir) {

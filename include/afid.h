@@ -164,6 +164,23 @@ int afid_global_quantities(afid_ctx *ctx, double *out11);
 int afid_calc_dissipation_nu(afid_ctx *ctx, double *nu_mu, double *nu_th, double *disste,
                              double *dissth);
 
+/* ---- statistics and slabs (StatRoutines.F90, SlabDumpRoutines.F90) ---------- */
+/* CalcStats (StatRoutines.F90:12-44): adds this sample's plane averages over
+ * (j, i) of vx, vy, vz, temp, their squares and temp*vx into the context's
+ * statistics (device reduction); nstatsamples += 1. */
+int afid_calc_stats(afid_ctx *ctx);
+/* The accumulated statistics, out[q*nxm + k-1] for q = vx_mean, vy_mean,
+ * vz_mean, temp_mean, vx_rms, vy_rms, vz_rms, temp_rms, tempvx_mean, and the
+ * dissipation profiles disste, dissth summed over the afid_calc_dissipation_nu
+ * calls (CalcDissipationNu.F90:101-104): 11*nxm values; *nsamples receives
+ * the number of afid_calc_stats calls. */
+int afid_get_stats(afid_ctx *ctx, double *out, int *nsamples);
+int afid_reset_stats(afid_ctx *ctx);
+/* SlabDumper (SlabDumpRoutines.F90:20-31): the planes k = kslab[m] (1-based,
+ * m < nslab) of (vx(k)+vx(k+1))/2, vy, vz and temp, as
+ * out[((m*4 + q)*nzm + i-1)*nym + j-1] (4*nslab*nym*nzm values). */
+int afid_get_slabs(afid_ctx *ctx, const int *kslab, int nslab, double *out);
+
 /* ---- profiling ----------------------------------------------------------------
  * When enabled, every kernel launch is bracketed by HIP events on the
  * context's stream; totals are per kernel name. */

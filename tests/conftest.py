@@ -1,4 +1,7 @@
 import os
+import re
+import shutil
+import subprocess
 import sys
 
 import numpy as np
@@ -18,6 +21,18 @@ def rel_l2(a, b):
     b = np.asarray(b).ravel()
     nb = np.linalg.norm(b)
     return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+H5DUMP = shutil.which("h5dump") or "/opt/conda/bin/h5dump"
+
+
+def h5_values(path, dset):
+    """Values of one HDF5 dataset, in file order, read with h5dump."""
+    if not os.path.exists(H5DUMP):
+        pytest.skip("h5dump not available")
+    out = subprocess.run([H5DUMP, "-m", "%.17g", "-d", dset, str(path)], capture_output=True, text=True, check=True).stdout
+    data = re.sub(r"\(\d+(,\d+)*\):", " ", out[out.index("DATA {") + 6:])
+    return np.array([float(v) for v in data.replace("}", " ").replace(",", " ").split()])
 
 
 def demean(a):

@@ -5,7 +5,9 @@ The host follows main.F90's loop (ntst+1 steps, CFL-limited dt with dtmin for
 the first two steps) and ends with errorcode 555; the final fields it writes
 must match the oracle stepped with the same dt logic to rel-L2 <= 1e-11
 (pr modulo its mean), through both the fused substep and the per-routine
-entry points.
+entry points.  With STATON and SLABDUMP set the host also writes
+stafield_master.h5 (one sample per output time after tsta), stst3locs.out and
+the stst3/ slab files at each output time.
 """
 import ctypes
 import os
@@ -14,7 +16,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from conftest import demean, rel_l2
+from conftest import demean, h5_values, rel_l2
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 RUN = os.path.join(ROOT, "afid_amd", "lib", "afid_run")
@@ -36,17 +38,17 @@ BOU = """ NXM  NYM  NZM   NSST(>1=RK,else AB) NREAD
  IDTV(0=dt fixed,1=var.)   DTMIN(for dt var.)  DTMAX(for dt var.)     VLIM
    {idtv}                       {dtmin}                {dtmax}                    1e3
  SLABDUMP(STST3)
-   0
+   {stst3}
 """
 
 CASE = dict(nxm=24, nym=16, nzm=20, istr3=4, str3=3.0, ylen=2.0, zlen=1.5, ray=1e6, pra=1.0)
 
 
-def write_bou(path, ntst=3, idtv=1, dt=2e-3, dtmin=1e-4, dtmax=5e-3, **c):
+def write_bou(path, ntst=3, idtv=1, dt=2e-3, dtmin=1e-4, dtmax=5e-3, stst3=0, **c):
     q = dict(CASE)
     q.update(c)
     with open(path, "w") as f:
-        f.write(BOU.format(ntst=ntst, idtv=idtv, dt=dt, dtmin=dtmin, dtmax=dtmax, **q))
+        f.write(BOU.format(ntst=ntst, idtv=idtv, dt=dt, dtmin=dtmin, dtmax=dtmax, stst3=stst3, **q))
 
 
 def need_run():
@@ -103,3 +105,27 @@ def test_fortran_main_loop_matches_oracle(tmp_path, routines):
     t = float(r.stdout.split("errorcode 555 time")[1].split()[0])
     assert t == pytest.approx(o.scalar("time"), rel=1e-12)
     assert os.path.exists(tmp_path / "nu_plate.out") and os.path.exists(tmp_path / "nu_vol.out")
+
+
+@pytest.mark.gpu
+def test_fortran_stats_and_slabs(tmp_path):
+    need_run()
+    from afid_amd import engine
+    from oracle.oracle import Oracle
+    write_bou(tmp_path / "bou.in", ntst=4, stst3=1)
+    xslab = [0.002, 0.004, 0.008, 0.02, 0.04, 0.08, 0.2, 0.5]
+    with open(tmp_path / "stst3.in", "w") as f:
+        f.write(" PROBEZ1\n " + "  ".join(str(v) for v in xslab) + "\n")
+    print("afid_run stats+slabs", flush=True)
+    r = subprocess.run([RUN, "bou.in", "--seed", "2"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "errorcode 555" in r.stdout
+    o = Oracle(nsst=3, alx3=1.0, **CASE)
+    xm = o.arr("xm")[1:CASE["nxm"] + 1]
+    ks = [int(v) for v in open(tmp_path / "stst3locs.out").read().split()]
+    assert ks == list(engine.slab_levels(xm, np.array(xslab)))
+    nout = len(open(tmp_path / "nu_plate.out").read().splitlines())
+    assert nout >= 2
+    assert h5_values(tmp_path / "stafield_master.h5", "averaging_time")[0] == nout
+    assert len(os.listdir(tmp_path / "stst3")) % 36 == 0 and len(os.listdir(tmp_path / "stst3")) >= 36
+    np.testing.assert_array_equal(h5_values(tmp_path / "stafield_master.h5", "X_cordin"), xm)
