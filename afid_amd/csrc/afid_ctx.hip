@@ -588,8 +588,10 @@ const char *afid_strerror(int code) {
     case AFID_ERR_STATE: return "call not valid in this state";
     case AFID_NUM_PIVOT: return "implicit factorisation needs a row interchange";
     case AFID_NUM_SINGULAR: return "singular pressure tridiagonal";
-    case AFID_ERR_IO: return "continuation file could not be read or written";
-    case AFID_ERR_GRID: return "continuation grid differs from this cell (regridding not available)";
+    case AFID_ERR_IO: return "continuation file could not be read or written, or holds a dataset of another shape";
+    case AFID_ERR_GRID:
+      return "old grid cannot be interpolated from (more than twice its resolution, a stretching other than 0 "
+             "or 6, or a point outside its window)";
     default: return "unknown error";
   }
 }
@@ -1379,6 +1381,131 @@ int afid_get_slabs(afid_ctx *c, const int *kslab, int nslab, double *out) {
   if (rc) return rc;
   HIPCHK(c, hipStreamSynchronize(c->st));
   HIPCHK(c, hipMemcpy(out, c->slab_buf, sizeof(double) * 4 * (size_t)nslab * c->g.ncol, hipMemcpyDeviceToHost));
+  return AFID_OK;
+}
+
+// ---- restart regridding ---------------------------------------------------
+// Grid of interp.F90's gridnew (:254-299): n faces rc and centres rm of
+// extent rext, uniform (istr 0) or clipped Chebyshev (istr 6, int(str)
+// nodes dropped at each end); rm(n) is extrapolated.  Other stretchings
+// are rejected: gridnew leaves rc undefined for them.
+static bool regrid_axis(int n, double rext, int istr, double str, std::vector<double> &rc,
+                        std::vector<double> &rm) {
+  rc.assign(n + 2, 0.0);
+  rm.assign(n + 2, 0.0);
+  if (n < 2) return false;
+  if (istr == 0) {
+    for (int i = 1; i <= n; i++) rc[i] = rext * ((double)(i - 1) / (double)(n - 1));
+  } else if (istr == 6) {
+    const int nclip = (int)str;
+    if (nclip < 0 || nclip > 200) return false;
+    const double pi = 2.0 * std::asin(1.0);
+    const int m = n + 2 * nclip;
+    std::vector<double> eta(n + 1);
+    for (int i = 1; i <= n; i++) eta[i] = std::cos(pi * ((double)(i + nclip) - 0.5) / (double)m);
+    const double half = 0.5 * (eta[1] - eta[n]);
+    for (int i = 1; i <= n; i++) eta[i] = eta[i] / half;
+    rc[1] = 0.0;
+    for (int i = 2; i < n; i++) rc[i] = rext * (1. - eta[i]) * 0.5;
+    rc[n] = rext;
+  } else {
+    return false;
+  }
+  for (int i = 1; i < n; i++) rm[i] = (rc[i] + rc[i + 1]) * 0.5;
+  rm[n] = 2 * rc[n] - rm[n - 1];
+  return true;
+}
+
+// Old axes get the linear end extrapolation of interp.F90:47-62.
+static void extend_axis(std::vector<double> &v, int n) {
+  v[0] = 2 * v[1] - v[2];
+  v[n + 1] = 2 * v[n] - v[n - 1];
+}
+
+int afid_interp_field(afid_ctx *c, int field, const double *old, int nxo, int nyo, int nzo, int istro3,
+                      double stro3) {
+  if (!c || !old) return AFID_ERR_ARG;
+  int intvar;
+  switch (field) {
+    case AFID_VX: intvar = 1; break;
+    case AFID_VY: intvar = 2; break;
+    case AFID_VZ: intvar = 3; break;
+    case AFID_TEMP: intvar = 4; break;
+    default: return AFID_ERR_ARG;
+  }
+  const Geom &g = c->g;
+  const int nx = g.nx, nym = g.nym, nzm = g.nzm, ny = nym + 1, nz = nzm + 1;
+  if (nxo < 2 || nyo < 3 || nzo < 3) return AFID_ERR_ARG;
+  if (nz > nzo * 2 || ny > nyo * 2 || nx > nxo * 2) return AFID_ERR_GRID;   // ReadFlowField.F90:72-82
+  const int nyom = nyo - 1, nzom = nzo - 1;
+  // one-rank window of ReadFlowField.F90:91-99 (integer divisions)
+  const int js = std::max(nyom / nym, 1), je = std::min(nym * nyom / nym, nyom);
+  const int is = std::max(nzom / nzm, 1), ie = std::min(nzm * nzom / nzm, nzom);
+  if (je - js < 1 || ie - is < 1) return AFID_ERR_GRID;
+  std::vector<double> tco, tmo, rco, rmo, zzo, zmo, tcn, tmn, rcn, rmn, zzn, zmn;
+  if (!regrid_axis(nzo, c->p.zlen, 0, 1.0, tco, tmo) || !regrid_axis(nyo, c->p.ylen, 0, 1.0, rco, rmo) ||
+      !regrid_axis(nxo, 1.0, istro3, stro3, zzo, zmo) || !regrid_axis(nz, c->p.zlen, 0, 1.0, tcn, tmn) ||
+      !regrid_axis(ny, c->p.ylen, 0, 1.0, rcn, rmn) || !regrid_axis(nx, 1.0, c->p.istr3, c->p.str3, zzn, zmn))
+    return AFID_ERR_GRID;
+  for (auto *v : {&tco, &tmo}) extend_axis(*v, nzo);
+  for (auto *v : {&rco, &rmo}) extend_axis(*v, nyo);
+  for (auto *v : {&zzo, &zmo}) extend_axis(*v, nxo);
+  // interp.F90:73-101: its "x" is the third index (z), its "z" the first (x)
+  const std::vector<double> &xo = intvar == 1 ? tco : tmo, &xn = intvar == 1 ? tcn : tmn;
+  const std::vector<double> &yo = intvar == 2 ? rco : rmo, &yn = intvar == 2 ? rcn : rmn;
+  const std::vector<double> &zo = intvar == 3 ? zzo : zmo, &zn = intvar == 3 ? zzn : zmn;
+  const int ne = nx + nym + nzm;
+  std::vector<int> lohi(2 * (size_t)ne);
+  std::vector<double> crd(3 * (size_t)ne);
+  // first l in 1..n with old(l) >= x brackets x as (l-1, l); (n-1, n) when none
+  auto fill = [&](int e0, int cnt, const std::vector<double> &on, int n, const std::vector<double> &nw, int lo_min,
+                  int hi_max) {
+    for (int q = 0; q < cnt; q++) {
+      const double x = nw[q + 1];
+      int lo = n - 1, hi = n;
+      for (int l = 1; l <= n; l++)
+        if (on[l] >= x) {
+          lo = l - 1;
+          hi = l;
+          break;
+        }
+      if (lo < lo_min || hi > hi_max) return false;   // outside the reference's old window
+      const int e = e0 + q;
+      lohi[2 * e] = lo;
+      lohi[2 * e + 1] = hi;
+      crd[3 * e] = x;
+      crd[3 * e + 1] = on[lo];
+      crd[3 * e + 2] = on[hi];
+    }
+    return true;
+  };
+  if (!fill(0, nx, zo, nxo, zn, 0, nxo + 1) || !fill(nx, nym, yo, nyo, yn, js - 1, je + 1) ||
+      !fill(nx + nym, nzm, xo, nzo, xn, is - 1, ie + 1))
+    return AFID_ERR_GRID;
+  const size_t nold = (size_t)nxo * nyom * nzom;
+  const size_t b_old = nold * sizeof(double), b_crd = crd.size() * sizeof(double);
+  const size_t b_lohi = lohi.size() * sizeof(int);
+  int rc = ensure_stage(c, b_old + b_crd + b_lohi);
+  if (rc) return rc;
+  char *st = (char *)c->stage;
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  HIPCHK(c, hipMemcpy(st, old, b_old, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(st + b_old, crd.data(), b_crd, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(st + b_old + b_crd, lohi.data(), b_lohi, hipMemcpyHostToDevice));
+  InterpArgs a;
+  a.g = g;
+  a.out = c->f[field];
+  a.old = (const double *)st;
+  a.nxo = nxo;
+  a.nyom = nyom;
+  a.js = js;
+  a.je = je;
+  a.is = is;
+  a.ie = ie;
+  a.crd = (const double *)(st + b_old);
+  a.lohi = (const int *)(st + b_old + b_crd);
+  if ((rc = kerr(c, launch_interp(a, c->st)))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->st));
   return AFID_OK;
 }
 

@@ -419,7 +419,8 @@ __global__ void __launch_bounds__(WG)
     for (int t = 0; t < LM; t++) {
       if (t < cntx) {
         const int k = k0 + t, s = t * P + p;
-        stf(vyn, n.C + (OffT)s * sizeof(double), k < nxm ? b[t] + fh[s] * cin : 0.0);
+        // row nx is outside the nxm system: the reference never writes it
+        stf(vyn, n.C + (OffT)s * sizeof(double), k < nxm ? b[t] + fh[s] * cin : ldf(vy, n.C + (OffT)s * sizeof(double)));
       }
     }
   }
@@ -469,7 +470,7 @@ __global__ void __launch_bounds__(WG)
     for (int t = 0; t < LM; t++) {
       if (t < cntx) {
         const int k = k0 + t, s = t * P + p;
-        const double y = k < nxm ? b[t] + fh[s] * cin : 0.0;
+        const double y = k < nxm ? b[t] + fh[s] * cin : ldf(vz, n.C + (OffT)s * sizeof(double));
         if (t == 0 && k == 0) dvz0 = y - vz0;
         stf(vzn, n.C + (OffT)s * sizeof(double), y);
       }
@@ -828,7 +829,8 @@ __global__ void __launch_bounds__(SOLVE_WG)
       for (int t = 0; t < LM; t++)
         if (t < L) {
           const int k = k0 + t, s = t * P + p;
-          stf(vyn, C + (OffT)s * sizeof(double), k < nxm ? b[t] + fh[s] * cin : 0.0);
+          // in place: row nx, outside the nxm system, is left as the reference leaves it
+          if (k != nx - 1) stf(vyn, C + (OffT)s * sizeof(double), k < nxm ? b[t] + fh[s] * cin : 0.0);
         }
   }
   load_chunk<LM, OffT>(b, bz, C, cntm, L, p);
@@ -843,7 +845,7 @@ __global__ void __launch_bounds__(SOLVE_WG)
       for (int t = 0; t < LM; t++)
         if (t < L) {
           const int k = k0 + t, s = t * P + p;
-          stf(vzn, C + (OffT)s * sizeof(double), k < nxm ? b[t] + fh[s] * cin : 0.0);
+          if (k != nx - 1) stf(vzn, C + (OffT)s * sizeof(double), k < nxm ? b[t] + fh[s] * cin : 0.0);
         }
   }
   dvz0 = __shfl(dvz0, 0, P);

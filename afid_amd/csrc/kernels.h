@@ -126,6 +126,20 @@ int reduce_partial_blocks();
 constexpr int STATS_N = 9;   // vx, vy, vz, T means; vx^2, vy^2, vz^2, T^2; T*vx
 hipError_t launch_stats(const ReduceArgs &a, double *acc, hipStream_t st);   // acc[STATS_N][nxm] +=
 size_t stats_partial_doubles(const Geom &g);
+// Restart regridding (interp.F90:11-208).  The old field as the
+// continuation file holds it, old[((i-1)*nyom + j-1)*nxo + k-1], read
+// through the padded view arrold(0:nxo+1, js-1:je+1, is-1:ie+1) of the
+// reference; per new k, j, i the bracketing old indices (lo, hi) and the
+// coordinates (x, x1, x2), laid out k (nx), then j (nym), then i (nzm).
+struct InterpArgs {
+  Geom g;
+  double *out;
+  const double *old;
+  int nxo, nyom, js, je, is, ie;
+  const int *lohi;      // 2 per entry
+  const double *crd;    // 3 per entry
+};
+hipError_t launch_interp(const InterpArgs &a, hipStream_t st);
 hipError_t launch_slabs(const FieldPtrs &f, const Geom &g, const int *kslab, int nslab, double *out,
                         hipStream_t st);
 

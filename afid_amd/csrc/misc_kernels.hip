@@ -254,6 +254,76 @@ hipError_t launch_slabs(const FieldPtrs &f, const Geom &g, const int *kslab, int
   return hipGetLastError();
 }
 
+// Restart regridding, interp + interptrilin (interp.F90:103-252), one
+// thread per new point.  The padded old array of the reference is read
+// through old_at: interior points from the file array, halo faces
+// extrapolated as 2 a(1) - a(2) (interp.F90:103-124), and halo edges and
+// corners, which the reference leaves unset, as 0 (a fresh allocation).
+__device__ inline double old_raw(const InterpArgs &a, int k, int j, int i) {
+  return a.old[((int64_t)(i - 1) * a.nyom + (j - 1)) * a.nxo + (k - 1)];
+}
+__device__ inline double old_at(const InterpArgs &a, int k, int j, int i) {
+  const int hk = (k == 0) | (k == a.nxo + 1);
+  const int hj = (j == a.js - 1) | (j == a.je + 1);
+  const int hi = (i == a.is - 1) | (i == a.ie + 1);
+  if (hk + hj + hi >= 2) return 0.0;
+  if (hk) {
+    const int k1 = k == 0 ? 1 : a.nxo, k2 = k == 0 ? 2 : a.nxo - 1;
+    return 2.0 * old_raw(a, k1, j, i) - old_raw(a, k2, j, i);
+  }
+  if (hi) {
+    const int i1 = i == a.is - 1 ? a.is : a.ie, i2 = i == a.is - 1 ? a.is + 1 : a.ie - 1;
+    return 2.0 * old_raw(a, k, j, i1) - old_raw(a, k, j, i2);
+  }
+  if (hj) {
+    const int j1 = j == a.js - 1 ? a.js : a.je, j2 = j == a.js - 1 ? a.js + 1 : a.je - 1;
+    return 2.0 * old_raw(a, k, j1, i) - old_raw(a, k, j2, i);
+  }
+  return old_raw(a, k, j, i);
+}
+__global__ void __launch_bounds__(256) k_interp(InterpArgs a) {
+#pragma clang fp contract(off)
+  const Geom g = a.g;
+  const int64_t n = g.ncol * (int64_t)g.nx;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = idx / g.nx;
+    const int k = (int)(idx - c * g.nx);
+    const int i = (int)(c / g.nym), j = (int)(c - (int64_t)i * g.nym);
+    const int ek = k, ej = g.nx + j, ei = g.nx + g.nym + i;
+    const int bifk = a.lohi[2 * ek], bick = a.lohi[2 * ek + 1];
+    const int bifj = a.lohi[2 * ej], bicj = a.lohi[2 * ej + 1];
+    const int bifi = a.lohi[2 * ei], bici = a.lohi[2 * ei + 1];
+    const double biz = a.crd[3 * ek], biz1 = a.crd[3 * ek + 1], biz2 = a.crd[3 * ek + 2];
+    const double biy = a.crd[3 * ej], biy1 = a.crd[3 * ej + 1], biy2 = a.crd[3 * ej + 2];
+    const double bix = a.crd[3 * ei], bix1 = a.crd[3 * ei + 1], bix2 = a.crd[3 * ei + 2];
+    // the eight corners as interp.F90:182-189 picks them: all but one of the
+    // upper-x corners come from the lower x level
+    const double aficjfk = old_at(a, bifk, bicj, bifi);
+    const double afifjfk = old_at(a, bifk, bifj, bifi);
+    const double acicjfk = old_at(a, bifk, bicj, bici);
+    const double acifjfk = old_at(a, bifk, bifj, bici);
+    const double aficjck = old_at(a, bick, bicj, bifi);
+    const double afifjck = afifjfk;
+    const double acicjck = acicjfk;
+    const double acifjck = acifjfk;
+    const double dxdydz = 1.0 / ((bix2 - bix1) * (biy2 - biy1) * (biz2 - biz1));
+    const double ans = (aficjfk * (bix2 - bix) * (biy - biy1) * (biz2 - biz) +
+                        afifjfk * (bix2 - bix) * (biy2 - biy) * (biz2 - biz) +
+                        acicjfk * (bix - bix1) * (biy - biy1) * (biz2 - biz) +
+                        acifjfk * (bix - bix1) * (biy2 - biy) * (biz2 - biz) +
+                        aficjck * (bix2 - bix) * (biy - biy1) * (biz - biz1) +
+                        afifjck * (bix2 - bix) * (biy2 - biy) * (biz - biz1) +
+                        acicjck * (bix - bix1) * (biy - biy1) * (biz - biz1) +
+                        acifjck * (bix - bix1) * (biy2 - biy) * (biz - biz1)) *
+                       dxdydz;
+    a.out[c * g.ldx + pos_of_k(k, g.L)] = ans;
+  }
+}
+hipError_t launch_interp(const InterpArgs &a, hipStream_t st) {
+  hipLaunchKernelGGL(k_interp, dim3(8192), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
 static hipError_t finalize(const ReduceArgs &a, int nq, int maxmask, int minmask, hipStream_t st) {
   hipLaunchKernelGGL(k_finalize, dim3(nq), dim3(256), 0, st, a.partial, a.out, nq, (const int *)nullptr,
                      maxmask, minmask);

@@ -81,6 +81,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "afid_get_stats": ([vp, pd, ctypes.POINTER(i)], i),
         "afid_reset_stats": ([vp], i),
         "afid_get_slabs": ([vp, ctypes.POINTER(i), i, pd], i),
+        "afid_interp_field": ([vp, i, pd, i, i, i, i, d], i),
         "afid_set_stale": ([vp, pd], i),
         "afid_calc_max_cfl": ([vp, pd], i),
         "afid_check_divergence": ([vp, pd], i),
@@ -375,6 +376,15 @@ class Cell:
 
     def ReadFlowField(self, directory: str = "."):
         self._check(load_io_library().afid_read_continuation(self.h, directory.encode()), "ReadFlowField")
+
+    def interp(self, name: str, old: np.ndarray, nxo: int, nyo: int, nzo: int, istro3: int, stro3: float):
+        """Restart regridding of one field (interp.F90): old is C (nzo-1, nyo-1, nxo) as the
+        continuation file holds it, i.e. Fortran (nxo, nyo-1, nzo-1)."""
+        old = np.ascontiguousarray(old, dtype=np.float64)
+        if old.shape != (nzo - 1, nyo - 1, nxo):
+            raise ValueError(f"old field shape {old.shape} is not {(nzo - 1, nyo - 1, nxo)}")
+        self._check(self.L.afid_interp_field(self.h, FIELD_IDS[name], _dp(old), nxo, nyo, nzo, istro3, stro3),
+                    f"interp {name}")
 
     # ---- diagnostics -------------------------------------------------------------
     def CalcMaxCFL(self) -> float:

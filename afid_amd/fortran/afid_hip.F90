@@ -255,6 +255,15 @@
           import :: c_int, c_ptr
           type(c_ptr), value :: ctx
         end function
+!       restart regridding (interp.F90)
+        integer(c_int) function afid_interp_field(ctx, field, old, nxo,  &
+     &      nyo, nzo, istro3, stro3) bind(C, name='afid_interp_field')
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: ctx
+          integer(c_int), value :: field, nxo, nyo, nzo, istro3
+          real(c_double), intent(in) :: old(*)
+          real(c_double), value :: stro3
+        end function
 !       statistics and slabs (StatRoutines.F90, SlabDumpRoutines.F90)
         integer(c_int) function afid_calc_stats(ctx)                    &
      &      bind(C, name='afid_calc_stats')

@@ -56,7 +56,7 @@ int read_dataset(hid_t f, const char *dset, hid_t type, void *buf, int rank, con
   if (sp < 0 || H5Sget_simple_extent_ndims(sp) != rank || H5Sget_simple_extent_dims(sp, got, nullptr) != rank)
     rc = AFID_ERR_IO;
   for (int q = 0; rc == AFID_OK && q < rank; q++)
-    if (got[q] != dims[q]) rc = AFID_ERR_GRID;
+    if (got[q] != dims[q]) rc = AFID_ERR_IO;   // dataset of another shape
   if (rc == AFID_OK && H5Dread(ds, type, H5S_ALL, H5S_ALL, H5P_DEFAULT, buf) < 0) rc = AFID_ERR_IO;
   if (sp >= 0) H5Sclose(sp);
   H5Dclose(ds);
@@ -295,15 +295,29 @@ int afid_read_continuation(afid_ctx *ctx, const char *dir) {
   if ((rc = afid_get_metric(ctx, "str3", v, 1))) return rc;
   const double str3 = v[0];
   if ((rc = afid_get_metric(ctx, "istr3", v, 1))) return rc;
-  if (nxo != d.nx || nyo != d.nym + 1 || nzo != d.nzm + 1 || istro3 != (int)v[0] || std::fabs(stro3 - str3) > 1e-8)
-    return AFID_ERR_GRID;
+  std::vector<double> plane((size_t)d.nym * d.nzm, 0.0);
+  if (nxo != d.nx || nyo != d.nym + 1 || nzo != d.nzm + 1 || istro3 != (int)v[0] || std::fabs(stro3 - str3) > 1e-8) {
+    // another grid: read each field on its own grid and interpolate
+    // (ReadFlowField.F90:64-158); pr, the histories and the stale row start
+    // from zero, as the reference restarts
+    if (nxo < 2 || nyo < 3 || nzo < 3) return AFID_ERR_IO;
+    std::vector<double> o((size_t)nxo * (nyo - 1) * (nzo - 1));
+    for (const FieldFile &ff : kReferenceFields) {
+      if ((rc = read_field_file(join(dir, ff.file), ff.dset, o.data(), nxo, nyo - 1, nzo - 1))) return rc;
+      if ((rc = afid_interp_field(ctx, ff.field, o.data(), nxo, nyo, nzo, istro3, stro3))) return rc;
+    }
+    std::vector<double> z((size_t)d.nx * d.nym * d.nzm, 0.0);
+    for (const FieldFile &ff : kStateFields)
+      if ((rc = afid_upload(ctx, ff.field, z.data(), d.nx, d.nym, 1, 1))) return rc;
+    if ((rc = afid_set_stale(ctx, plane.data()))) return rc;
+    return afid_set_time(ctx, time);
+  }
   std::vector<double> a((size_t)d.nx * d.nym * d.nzm);
   for (const FieldFile &ff : kReferenceFields) {
     if ((rc = read_field_file(join(dir, ff.file), ff.dset, a.data(), d.nx, d.nym, d.nzm))) return rc;
     if ((rc = afid_upload(ctx, ff.field, a.data(), d.nx, d.nym, 1, 1))) return rc;
   }
   const std::string state = join(dir, "continua_state.h5");
-  std::vector<double> plane((size_t)d.nym * d.nzm, 0.0);
   hid_t fs = H5Fopen(state.c_str(), H5F_ACC_RDONLY, H5P_DEFAULT);
   if (fs >= 0) {
     H5Fclose(fs);

@@ -28,9 +28,10 @@ extern "C" {
 #define AFID_ERR_NOMEM (-3)      /* device allocation failed                 */
 #define AFID_ERR_FFTSIZE (-4)    /* lateral size has a prime factor > 5 (errorcode 444) */
 #define AFID_ERR_STATE (-5)      /* call not valid in the context's state    */
-#define AFID_ERR_IO (-6)         /* continuation file missing or unreadable (afid_io.h) */
-#define AFID_ERR_GRID (-7)       /* continuation written on another grid; the reference
-                                    would interpolate (ReadFlowField.F90:56-141) */
+#define AFID_ERR_IO (-6)         /* continuation file missing, unreadable or of another
+                                    shape (afid_io.h) */
+#define AFID_ERR_GRID (-7)       /* continuation grid that cannot be interpolated from
+                                    (ReadFlowField.F90:72-82, interp.F90:254-299) */
 #define AFID_NUM_PIVOT 1         /* an implicit factorisation would need a row
                                     interchange (dgttrf pivoting) */
 #define AFID_NUM_SINGULAR 2      /* singular pressure tridiagonal (zgttrf info>0),
@@ -180,6 +181,21 @@ int afid_reset_stats(afid_ctx *ctx);
  * m < nslab) of (vx(k)+vx(k+1))/2, vy, vz and temp, as
  * out[((m*4 + q)*nzm + i-1)*nym + j-1] (4*nslab*nym*nzm values). */
 int afid_get_slabs(afid_ctx *ctx, const int *kslab, int nslab, double *out);
+
+/* ---- restart regridding (interp.F90, ReadFlowField.F90:56-158) -------------- */
+/* interp (interp.F90:11-208): sets field (AFID_VX, AFID_VY, AFID_VZ or
+ * AFID_TEMP; intvar 1..4) on this grid from the field of an old grid with nxo,
+ * nyo, nzo points (continua_master.h5's nx, ny, nz) and stretching istro3,
+ * stro3.  old is the continuation array, Fortran (nxo, nyo-1, nzo-1).  The old
+ * window is ReadFlowField's at one rank (:91-99); the trilinear weights and
+ * the corner values are those of interp.F90:128-207 as written.  Halo edges
+ * and corners of the old array, which the reference leaves unset, read as 0.
+ * AFID_ERR_GRID: more than twice the old resolution (ReadFlowField.F90:72-82),
+ * a stretching other than 0 or 6 (gridnew handles no other), or a point whose
+ * bracket falls outside the old window (the reference would read out of
+ * bounds). */
+int afid_interp_field(afid_ctx *ctx, int field, const double *old, int nxo, int nyo, int nzo, int istro3,
+                      double stro3);
 
 /* ---- profiling ----------------------------------------------------------------
  * When enabled, every kernel launch is bracketed by HIP events on the

@@ -12,9 +12,14 @@
  * (ReadFlowField.F90:160-170).  With them a restart continues bit for bit; a
  * directory without that file restarts as the reference does.
  *
+ * When continua_master.h5 names another grid (ReadFlowField.F90:64-66),
+ * the reader interpolates each field onto this grid with afid_interp_field
+ * (ReadFlowField.F90:67-158) and starts pr, the histories and the stale row
+ * from zero, as the reference does.
+ *
  * Return codes as in afid.h; AFID_ERR_IO for a missing or malformed file,
- * AFID_ERR_GRID when continua_master.h5 names another grid (the reference
- * would interpolate, ReadFlowField.F90:56-141).
+ * AFID_ERR_GRID for an old grid that cannot be interpolated from (see
+ * afid_interp_field); the fields may then be partly set.
  */
 #ifndef AFID_IO_H
 #define AFID_IO_H
@@ -25,8 +30,9 @@ extern "C" {
 
 /* WriteFlowField.F90:11-45 into directory dir (time from afid_get_time). */
 int afid_write_continuation(afid_ctx *ctx, const char *dir);
-/* ReadFlowField.F90:29-45, 160-170 from directory dir: fields, time
- * (afid_set_time), and the state file when present. */
+/* ReadFlowField.F90:29-170 from directory dir: fields (interpolated from
+ * another grid), time (afid_set_time), and the state file when present and
+ * the grid is the same. */
 int afid_read_continuation(afid_ctx *ctx, const char *dir);
 
 /* WriteStats (StatRoutines.F90:49-137): stafield_master.h5 in dir with the

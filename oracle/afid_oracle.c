@@ -1269,3 +1269,169 @@ void afo_calc_dissipation_nu(afo_t *o, double *nu_mu_out, double *nu_th_out, dou
   *nu_th_out = nu_th * volt;
   free(dxm); free(dxc); free(lem); free(lec);
 }
+
+/* ---- restart regridding -------------------------------------------------- */
+
+/* gridnew (interp.F90:254-299), 0-based outputs.  Uniform faces for istr 0;
+ * for istr 6 the clipped Chebyshev nodes, with nclip = int(str) nodes
+ * dropped at each end of n + 2 nclip.  Centres are face midpoints and the
+ * last one is extrapolated, rm(n) = 2 rc(n) - rm(n-1). */
+int afo_gridnew(int n, double rext, int istr, double str, double *rc, double *rm) {
+  if (istr == 0) {
+    for (int i = 1; i <= n; i++) {
+      double x3 = (double)(i - 1) / (double)(n - 1);
+      rc[i - 1] = rext * x3;
+    }
+  } else if (istr == 6) {
+    const double pi = 2.0 * asin(1.0);
+    const int nclip = (int)str;
+    if (nclip < 0 || nclip > 200) return 2;        /* etazm(1:n+400) */
+    const int nxmo = n + nclip + nclip;
+    double *etaz = (double *)malloc(sizeof(double) * (size_t)(n + 1));
+    for (int i = 1; i <= n; i++) {
+      const int ii = i + nclip;
+      etaz[i] = +cos(pi * ((double)ii - 0.5) / (double)nxmo);
+    }
+    const double delet = etaz[1] - etaz[n];
+    for (int i = 1; i <= n; i++) etaz[i] = etaz[i] / (0.5 * delet);
+    rc[0] = 0.0;
+    for (int i = 2; i <= n - 1; i++) rc[i - 1] = rext * (1. - etaz[i]) * 0.5;
+    rc[n - 1] = rext;
+    free(etaz);
+  } else {
+    return 2;
+  }
+  for (int i = 1; i <= n - 1; i++) rm[i - 1] = (rc[i - 1] + rc[i]) * 0.5;
+  rm[n - 1] = 2 * rc[n - 1] - rm[n - 2];
+  return 0;
+}
+
+/* Old grid, 0..n+1 with the second-order end extrapolation
+ * (interp.F90:47-62). */
+static int old_axis(int n, double rext, int istr, double str, double *c, double *m) {
+  int rc = afo_gridnew(n, rext, istr, str, c + 1, m + 1);
+  if (rc) return rc;
+  c[0] = 2 * c[1] - c[2];
+  c[n + 1] = 2 * c[n] - c[n - 1];
+  m[0] = 2 * m[1] - m[2];
+  m[n + 1] = 2 * m[n] - m[n - 1];
+  return 0;
+}
+
+/* First l in 1..n with old[l] >= x gives (l-1, l); (n-1, n) when none
+ * (interp.F90:139-170). */
+static void bracket(const double *old, int n, double x, int *lo, int *hi) {
+  *lo = n - 1;
+  *hi = n;
+  for (int l = 1; l <= n; l++)
+    if (old[l] >= x) {
+      *lo = l - 1;
+      *hi = l;
+      return;
+    }
+}
+
+int afo_interp(const double *old, int nxo, int nyo, int nzo, int istro3, double stro3,
+               int nxm, int nym, int nzm, int istr3, double str3, double ylen, double zlen,
+               int intvar, double *out) {
+  const int nx = nxm + 1, ny = nym + 1, nz = nzm + 1;
+  const int nyom = nyo - 1, nzom = nzo - 1;
+  if (nz > nzo * 2 || ny > nyo * 2 || nx > nxo * 2) return 1;
+  if (intvar < 1 || intvar > 4) return 2;
+  /* window of ReadFlowField.F90:91-99 at one rank (xstart = 1, xend = n):
+   * floor(real(1*nyom/nym)) is an integer division */
+  int js = nyom / nym, je = nym * nyom / nym, is = nzom / nzm, ie = nzm * nzom / nzm;
+  if (js < 1) js = 1;
+  if (je > nyom) je = nyom;
+  if (is < 1) is = 1;
+  if (ie > nzom) ie = nzom;
+  if (je - js < 1 || ie - is < 1 || nxo < 2) return 2;
+  double *tco = dvec(nzo), *tmo = dvec(nzo), *rco = dvec(nyo), *rmo = dvec(nyo), *zzo = dvec(nxo),
+         *zmo = dvec(nxo);
+  double *tcn = dvec(nz), *tmn = dvec(nz), *rcn = dvec(ny), *rmn = dvec(ny), *zzn = dvec(nx), *zmn = dvec(nx);
+  int rc = old_axis(nzo, zlen, 0, 1.0, tco, tmo);
+  if (!rc) rc = old_axis(nyo, ylen, 0, 1.0, rco, rmo);
+  if (!rc) rc = old_axis(nxo, 1.0, istro3, stro3, zzo, zmo);
+  if (!rc) rc = afo_gridnew(nz, zlen, 0, 1.0, tcn + 1, tmn + 1);
+  if (!rc) rc = afo_gridnew(ny, ylen, 0, 1.0, rcn + 1, rmn + 1);
+  if (!rc) rc = afo_gridnew(nx, 1.0, istr3, str3, zzn + 1, zmn + 1);
+  /* interp.F90:73-101: "x" is the third index (z), "z" the first (x) */
+  const double *xo = intvar == 1 ? tco : tmo, *xn = intvar == 1 ? tcn : tmn;
+  const double *yo = intvar == 2 ? rco : rmo, *yn = intvar == 2 ? rcn : rmn;
+  const double *zo = intvar == 3 ? zzo : zmo, *zn = intvar == 3 ? zzn : zmn;
+  /* padded old array arrold(0:nxo+1, js-1:je+1, is-1:ie+1), zero where the
+   * reference leaves it unset (edges and corners of the halo) */
+  const int n1 = nxo + 2, n2 = je - js + 3, n3 = ie - is + 3;
+  double *a = rc ? NULL : (double *)calloc((size_t)n1 * n2 * n3, sizeof(double));
+#define AO(k, j, i) a[(((size_t)((i) - (is - 1)) * n2 + ((j) - (js - 1))) * n1) + (k)]
+  if (a) {
+    for (int i = is; i <= ie; i++)
+      for (int j = js; j <= je; j++)
+        for (int k = 1; k <= nxo; k++) AO(k, j, i) = old[((size_t)(i - 1) * nyom + (j - 1)) * nxo + (k - 1)];
+    /* interp.F90:103-124 */
+    for (int i = is; i <= ie; i++)
+      for (int j = js; j <= je; j++) {
+        AO(0, j, i) = 2.0 * AO(1, j, i) - AO(2, j, i);
+        AO(nxo + 1, j, i) = 2.0 * AO(nxo, j, i) - AO(nxo - 1, j, i);
+      }
+    for (int j = js; j <= je; j++)
+      for (int k = 1; k <= nxo; k++) {
+        AO(k, j, is - 1) = 2.0 * AO(k, j, is) - AO(k, j, is + 1);
+        AO(k, j, ie + 1) = 2.0 * AO(k, j, ie) - AO(k, j, ie - 1);
+      }
+    for (int i = is; i <= ie; i++)
+      for (int k = 1; k <= nxo; k++) {
+        AO(k, js - 1, i) = 2.0 * AO(k, js, i) - AO(k, js + 1, i);
+        AO(k, je + 1, i) = 2.0 * AO(k, je, i) - AO(k, je - 1, i);
+      }
+  }
+  for (int i = 1; i <= nzm && !rc; i++) {
+    int bifi, bici;
+    const double bix = xn[i];
+    bracket(xo, nzo, bix, &bifi, &bici);
+    if (bifi < is - 1 || bici > ie + 1) rc = 3;
+    for (int j = 1; j <= nym && !rc; j++) {
+      int bifj, bicj;
+      const double biy = yn[j];
+      bracket(yo, nyo, biy, &bifj, &bicj);
+      if (bifj < js - 1 || bicj > je + 1) {
+        rc = 3;
+        break;
+      }
+      for (int k = 1; k <= nx; k++) {
+        int bifk, bick;
+        const double biz = zn[k];
+        bracket(zo, nxo, biz, &bifk, &bick);
+        const double bix1 = xo[bifi], bix2 = xo[bici];
+        const double biy1 = yo[bifj], biy2 = yo[bicj];
+        const double biz1 = zo[bifk], biz2 = zo[bick];
+        /* interp.F90:182-189, with its corner indices as written */
+        const double aficjfk = AO(bifk, bicj, bifi);
+        const double afifjfk = AO(bifk, bifj, bifi);
+        const double acicjfk = AO(bifk, bicj, bici);
+        const double acifjfk = AO(bifk, bifj, bici);
+        const double aficjck = AO(bick, bicj, bifi);
+        const double afifjck = AO(bifk, bifj, bifi);
+        const double acicjck = AO(bifk, bicj, bici);
+        const double acifjck = AO(bifk, bifj, bici);
+        /* interptrilin (interp.F90:210-252) */
+        const double dxdydz = 1.0 / ((bix2 - bix1) * (biy2 - biy1) * (biz2 - biz1));
+        out[((size_t)(i - 1) * nym + (j - 1)) * nx + (k - 1)] =
+            (aficjfk * (bix2 - bix) * (biy - biy1) * (biz2 - biz) +
+             afifjfk * (bix2 - bix) * (biy2 - biy) * (biz2 - biz) +
+             acicjfk * (bix - bix1) * (biy - biy1) * (biz2 - biz) +
+             acifjfk * (bix - bix1) * (biy2 - biy) * (biz2 - biz) +
+             aficjck * (bix2 - bix) * (biy - biy1) * (biz - biz1) +
+             afifjck * (bix2 - bix) * (biy2 - biy) * (biz - biz1) +
+             acicjck * (bix - bix1) * (biy - biy1) * (biz - biz1) +
+             acifjck * (bix - bix1) * (biy2 - biy) * (biz - biz1)) *
+            dxdydz;
+      }
+    }
+  }
+#undef AO
+  free(a);
+  free(tco); free(tmo); free(rco); free(rmo); free(zzo); free(zmo);
+  free(tcn); free(tmn); free(rcn); free(rmn); free(zzn); free(zmn);
+  return rc;
+}

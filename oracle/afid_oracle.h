@@ -88,6 +88,21 @@ void afo_dgttrs(int n, int nrhs, const double *dl, const double *d, const double
 
 void afo_set_threads(int n);
 
+/* Restart regridding: interp (interp.F90:11-208) of one field from a grid
+ * (nxo, nyo, nzo, istro3, stro3) onto (nxm+1, nym+1, nzm+1, istr3, str3),
+ * with the one-rank old window of ReadFlowField.F90:64-98.  old is the
+ * continuation array, Fortran (nxo, nyo-1, nzo-1); out is Fortran
+ * (nxm+1, nym, nzm).  intvar 1..4 = vx, vy, vz, temp.  Returns 0, or
+ * 1 more than twice the old resolution (ReadFlowField.F90:72-82),
+ * 2 a stretching other than 0 / 6 or a window narrower than 2,
+ * 3 a read outside the old window (the reference would read out of bounds). */
+int afo_interp(const double *old, int nxo, int nyo, int nzo, int istro3, double stro3,
+               int nxm, int nym, int nzm, int istr3, double str3, double ylen, double zlen,
+               int intvar, double *out);
+/* gridnew (interp.F90:254-299): rc[0..n-1], rm[0..n-1].  Returns 0, or 2 for
+ * a stretching other than 0 / 6. */
+int afo_gridnew(int n, double rext, int istr, double str, double *rc, double *rm);
+
 #ifdef __cplusplus
 }
 #endif

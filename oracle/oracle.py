@@ -73,6 +73,10 @@ def lib() -> ctypes.CDLL:
         L.afo_dgttrf.restype = i
         L.afo_dgttrs.argtypes = [i, i] + [ctypes.POINTER(d)] * 4 + [ctypes.POINTER(i), ctypes.POINTER(d), i]
         L.afo_set_threads.argtypes = [i]
+        L.afo_interp.argtypes = [ctypes.POINTER(d), i, i, i, i, d, i, i, i, i, d, d, d, i, ctypes.POINTER(d)]
+        L.afo_interp.restype = i
+        L.afo_gridnew.argtypes = [i, d, i, d, ctypes.POINTER(d), ctypes.POINTER(d)]
+        L.afo_gridnew.restype = i
         _lib = L
     return _lib
 
@@ -223,6 +227,30 @@ def fft_c2r(c: np.ndarray, n: int) -> np.ndarray:
     return out
 
 
+def gridnew(n: int, rext: float, istr: int, str_: float):
+    """interp.F90's gridnew: (rc, rm), n points each."""
+    rc, rm = np.zeros(n), np.zeros(n)
+    err = lib().afo_gridnew(n, rext, istr, str_, _dp(rc), _dp(rm))
+    if err:
+        raise ValueError(f"gridnew: stretching {istr} not handled")
+    return rc, rm
+
+
+def interp(old: np.ndarray, nxo: int, nyo: int, nzo: int, istro3: int, stro3: float, new: dict,
+           intvar: int):
+    """Restart regridding of one field (interp.F90 with ReadFlowField's one-rank window).
+
+    old: C (nzo-1, nyo-1, nxo) = Fortran (nxo, nyom, nzom); new: dict with nxm, nym, nzm,
+    istr3, str3, ylen, zlen.  Returns (code, out) with out C (nzm, nym, nxm+1)."""
+    old = np.ascontiguousarray(old, dtype=np.float64)
+    assert old.shape == (nzo - 1, nyo - 1, nxo)
+    q = new
+    out = np.zeros((q["nzm"], q["nym"], q["nxm"] + 1))
+    code = lib().afo_interp(_dp(old), nxo, nyo, nzo, istro3, stro3, q["nxm"], q["nym"], q["nzm"], q["istr3"],
+                            q["str3"], q["ylen"], q["zlen"], intvar, _dp(out))
+    return code, out
+
+
 # ---- reference harness (oracle/_ref/ref_harness, built only where the
 # ---- reference source tree is present) ------------------------------------
 def ref_available() -> bool:
@@ -243,3 +271,14 @@ def run_ref(workdir: str, params: dict, mode: int, nsteps: int = 1, dt: float = 
     env.setdefault("OMP_NUM_THREADS", "1")
     subprocess.run([REF_HARNESS], cwd=workdir, check=True, env=env, timeout=timeout,
                    stdout=subprocess.DEVNULL, stderr=subprocess.PIPE)
+
+
+def run_ref_interp(workdir: str, params: dict, old: np.ndarray, nxo: int, nyo: int, nzo: int, istro3: int,
+                   stro3: float, intvar: int, timeout: float = 120) -> np.ndarray:
+    """The reference interp on the new grid of params (mode 4 of the harness)."""
+    np.ascontiguousarray(old, dtype=np.float64).tofile(os.path.join(workdir, "interp_old.bin"))
+    with open(os.path.join(workdir, "interp.in"), "w") as f:
+        f.write(f"{nxo} {nyo} {nzo} {istro3} {stro3!r} {intvar}\n")
+    run_ref(workdir, params, mode=4, timeout=timeout)
+    out = np.fromfile(os.path.join(workdir, "interp_out.bin"), dtype=np.float64)
+    return out.reshape(params["nzm"], params["nym"], params["nxm"] + 1)

@@ -19,6 +19,10 @@
 !   mode nsteps dt idtv dtmin dtmax limitcfl
 ! mode 1: dump grid.  mode 2: per-routine stages from fields_in.bin.
 ! mode 3: march nsteps steps from fields_in.bin.
+! mode 4: the reference interp of interp_old.bin (Fortran (nxo, nyo-1, nzo-1))
+!         onto this grid; interp.in holds nxo nyo nzo istro3 stro3 intvar.
+!         The old window is ReadFlowField's at one rank; the padded old array
+!         starts at zero, as a fresh allocation does.
 
 module harness_glue
   use iso_c_binding
@@ -211,11 +215,40 @@ program ref_harness
     open(newunit=u, file='march_out.bin', access='stream', form='unformatted', status='replace')
     write(u) vx, vy, vz, temp, pr
     close(u)
+  else if (mode == 4) then
+    call run_interp()
   end if
   nu_diag = 0.d0
   call decomp_2d_finalize
 
 contains
+  subroutine run_interp()
+    integer :: uu, nxo, nyo, nzo, istro3, intvar, nyom, nzom, js, je, is, ie
+    real(8) :: stro3
+    real(8), allocatable :: old(:,:,:), arrold(:,:,:), arrnew(:,:,:)
+    open(newunit=uu, file='interp.in', status='old')
+    read(uu, *) nxo, nyo, nzo, istro3, stro3, intvar
+    close(uu)
+    nyom = nyo - 1
+    nzom = nzo - 1
+    allocate(old(nxo, nyom, nzom))
+    open(newunit=uu, file='interp_old.bin', access='stream', form='unformatted', status='old')
+    read(uu) old
+    close(uu)
+    js = max(floor(real(xstart(2)*nyom/nym)), 1)
+    je = min(ceiling(real(xend(2)*nyom/nym)), nyom)
+    is = max(floor(real(xstart(3)*nzom/nzm)), 1)
+    ie = min(ceiling(real(xend(3)*nzom/nzm)), nzom)
+    allocate(arrold(0:nxo+1, js-1:je+1, is-1:ie+1))
+    arrold = 0.d0
+    arrold(1:nxo, js:je, is:ie) = old(:, js:je, is:ie)
+    allocate(arrnew(nx, nym, nzm))
+    call interp(arrold, arrnew, nzo, nyo, nxo, istro3, stro3, intvar, js, je, is, ie)
+    open(newunit=uu, file='interp_out.bin', access='stream', form='unformatted', status='replace')
+    write(uu) arrnew
+    close(uu)
+  end subroutine
+
   subroutine read_fields()
     integer :: uu
     open(newunit=uu, file='fields_in.bin', access='stream', form='unformatted', status='old')

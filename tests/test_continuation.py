@@ -53,7 +53,7 @@ def test_field_file_errors(tmp_path):
     a = np.ones((2, 3, 4))
     path = str(tmp_path / "f.h5")
     engine.h5_write_field(path, "var", a)
-    with pytest.raises(engine.AfidError, match="other grid|regridding"):
+    with pytest.raises(engine.AfidError, match="another shape"):
         engine.h5_read_field(path, "var", (2, 3, 5))
     with pytest.raises(engine.AfidError, match="could not be read"):
         engine.h5_read_field(str(tmp_path / "missing.h5"), "var", a.shape)
@@ -117,13 +117,23 @@ def test_reference_style_restart_matches_oracle(tmp_path):
 
 
 @pytest.mark.gpu
-def test_continuation_on_another_grid_is_refused(tmp_path):
+def test_continuation_grid_that_cannot_be_interpolated_is_refused(tmp_path):
+    """gridnew (interp.F90:254-299) handles the stretchings 0 and 6 only, and
+    ReadFlowField.F90:72-82 stops at more than twice the old resolution."""
     engine = io_or_skip()
-    a = engine.Cell(engine.CellParams(**CASE))
+    a = engine.Cell(engine.CellParams(**CASE))        # istr3 = 4
     a.init_linear_noise(1, 1e-3)
     a.WriteFlowField(str(tmp_path))
     b = engine.Cell(engine.CellParams(**dict(CASE, nym=18)))
-    with pytest.raises(engine.AfidError, match="another grid|regridding"):
+    with pytest.raises(engine.AfidError, match="cannot be interpolated"):
+        b.ReadFlowField(str(tmp_path))
+    a.close()
+    b.close()
+    a = engine.Cell(engine.CellParams(**dict(CASE, istr3=6, str3=4.0)))
+    a.init_linear_noise(1, 1e-3)
+    a.WriteFlowField(str(tmp_path))
+    b = engine.Cell(engine.CellParams(**dict(CASE, istr3=6, str3=4.0, nzm=2 * CASE["nzm"] + 10)))
+    with pytest.raises(engine.AfidError, match="cannot be interpolated"):
         b.ReadFlowField(str(tmp_path))
     a.close()
     b.close()
