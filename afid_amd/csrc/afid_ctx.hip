@@ -81,6 +81,7 @@ struct afid_ctx {
   double *stage = nullptr;
   size_t stage_bytes = 0;
   size_t bytes_held = 0;
+  bool dev_steps_ready = false;   // device step state set and the step graph captured
   DevPlan dpy, dpz;
   RealPlan py{};
   FFTPlan pz{};
@@ -688,10 +689,19 @@ int afid_create(afid_ctx **out, const afid_params *p, const afid_metrics *m) {
     c->py.post = c->dpy.post;
     TRY(make_plan(c, g.nzm, &c->pz, &c->dpz, 0));
     if (c->py.H > 1280 || g.nzm > 2560) TRY(AFID_ERR_ARG);
+    // the dynamic-LDS limits are per kernel and process-wide: only ever raised,
+    // so that cells created earlier keep what they launch with
+    static size_t fft_lds_set = 65536, col_lds_set = 65536;
     size_t lds = std::max(sizeof(double2) * 8 * (size_t)c->py.H, sizeof(double2) * 4 * (size_t)g.nzm);
-    if (lds > 65536) TRYH(fft_configure_lds(lds));
+    if (lds > fft_lds_set) {
+      TRYH(fft_configure_lds(lds));
+      fft_lds_set = lds;
+    }
     size_t clds = std::max(sizeof(double) * 256 * (size_t)g.L, column_lds_need(g));
-    if (clds > 65536) TRYH(column_configure_lds(clds));
+    if (clds > col_lds_set) {
+      TRYH(column_configure_lds(clds));
+      col_lds_set = clds;
+    }
   }
   for (int q : {AFID_VX, AFID_VY, AFID_VZ, AFID_TEMP, AFID_PR, AFID_RUX, AFID_RUY, AFID_RUZ, AFID_RUTEMP, AFID_DPH})
     TRY(alloc_field(c, &c->f[q]));
@@ -701,6 +711,7 @@ int afid_create(afid_ctx **out, const afid_params *p, const afid_metrics *m) {
   else
     for (int q = 0; q < 4; q++) TRY(alloc_field(c, &c->fn[q]));
   TRYH(hipMalloc(&c->stale, sizeof(double) * g.ncol));
+  c->bytes_held += sizeof(double) * g.ncol;
   TRYH(hipMemsetAsync(c->stale, 0, sizeof(double) * g.ncol, c->st));
   {
     size_t ns = (size_t)g.nxm * g.nzm * c->ldm;
@@ -720,6 +731,7 @@ int afid_create(afid_ctx **out, const afid_params *p, const afid_metrics *m) {
     put(KC_AM3SK, c->am3sk, g.nx); put(KC_AC3SK, c->ac3sk, g.nx); put(KC_AP3SK, c->ap3sk, g.nx);
     put(KC_AMPHK, c->amphk, g.nxm); put(KC_ACPHK, c->acphk, g.nxm); put(KC_APPHK, c->apphk, g.nxm);
     TRYH(hipMalloc(&c->kc, sizeof(double) * rows.size()));
+    c->bytes_held += sizeof(double) * rows.size();
     TRYH(hipMemcpy(c->kc, rows.data(), sizeof(double) * rows.size(), hipMemcpyHostToDevice));
   }
   // natural-order arrays
@@ -754,6 +766,7 @@ int afid_create(afid_ctx **out, const afid_params *p, const afid_metrics *m) {
     add(lem.data(), nxm);
     add(lec.data(), nxm);
     TRYH(hipMalloc(&c->nat, sizeof(double) * nat.size()));
+    c->bytes_held += sizeof(double) * nat.size();
     TRYH(hipMemcpy(c->nat, nat.data(), sizeof(double) * nat.size(), hipMemcpyHostToDevice));
     c->d_g3rc = c->nat + off[0];
     c->d_xc = c->nat + off[1];
@@ -769,9 +782,11 @@ int afid_create(afid_ctx **out, const afid_params *p, const afid_metrics *m) {
   }
   TRYH(hipMalloc(&c->partial, sizeof(double) * std::max<int64_t>(reduce_partial_blocks() * 16, 2 * g.nxm + 8)));
   TRYH(hipMalloc(&c->out, sizeof(double) * (2 * g.nxm + 16)));
+  c->bytes_held += sizeof(double) * (std::max<int64_t>(reduce_partial_blocks() * 16, 2 * g.nxm + 8) + 2 * g.nxm + 16);
   TRYH(hipHostMalloc(&c->pin_fac, fac_bytes(c)));
   TRYH(hipMalloc(&c->fac_x, fac_bytes(c)));
   for (int ns = 1; ns <= 3; ns++) TRYH(hipMalloc(&c->fac_sub[ns], fac_bytes(c)));
+  c->bytes_held += 4 * fac_bytes(c);
   TRYH(hipStreamSynchronize(c->st));
 #undef TRY
 #undef TRYH
@@ -818,9 +833,11 @@ int afid_get_metric(afid_ctx *c, const char *name, double *out, int n) {
 static int ensure_stage(afid_ctx *c, size_t bytes) {
   if (c->stage_bytes >= bytes) return AFID_OK;
   hipFree(c->stage);
+  c->bytes_held -= c->stage_bytes;
   c->stage = nullptr;
   c->stage_bytes = 0;
   if (hipMalloc(&c->stage, bytes) != hipSuccess) return AFID_ERR_NOMEM;
+  c->bytes_held += bytes;
   c->stage_bytes = bytes;
   return AFID_OK;
 }
@@ -1181,29 +1198,16 @@ static int enqueue_step_device(afid_ctx *c) {
   return kerr(c, launch_step_end(c->dstep, c->st));
 }
 
-int afid_steps(afid_ctx *c, int ntime0, int nsteps, int idtv, double dt, double dtmin, double dtmax,
-               double limitcfl, double *dt_last, int *errorcode, int *steps_done) {
-  if (!c || nsteps < 0 || (!idtv && dt <= 0)) return AFID_ERR_ARG;
-  if (c->singular_mode) return AFID_NUM_SINGULAR;
-  if (!c->split) {   // the fused update swaps buffers per substep: step from the host
-    int err = 0, done = 0;
-    double dtu = dt;
-    for (int n = 0; n < nsteps && err == 0; n++, done++) {
-      int rc = afid_step(c, ntime0 + n, idtv, dt, dtmin, dtmax, limitcfl, &dtu, &err);
-      if (rc) return rc;
-    }
-    if (dt_last) *dt_last = dtu;
-    if (errorcode) *errorcode = err;
-    if (steps_done) *steps_done = done;
-    return AFID_OK;
-  }
+// Device step state from the host values, and the step graph, captured once.
+static int dev_steps_prepare(afid_ctx *c, int ntime0, int idtv, double dt, double dtmin, double dtmax,
+                             double limitcfl) {
   int rc = ensure_dev_step(c);
   if (rc) return rc;
   DevStep h{};
   h.time = c->time; h.ntime = ntime0; h.idtv = idtv; h.dtfix = dt; h.dtmin = dtmin; h.dtmax = dtmax;
   h.limitcfl = limitcfl; h.dt = dt;
+  HIPCHK(c, hipMemcpyAsync(c->dstep, &h, sizeof(DevStep), hipMemcpyHostToDevice, c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));
-  HIPCHK(c, hipMemcpy(c->dstep, &h, sizeof(DevStep), hipMemcpyHostToDevice));
   if (!c->step_graph) {   // capture one step once; its kernels take no per-step host data
     hipGraph_t graph = nullptr;
     HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
@@ -1220,14 +1224,116 @@ int afid_steps(afid_ctx *c, int ntime0, int nsteps, int idtv, double dt, double 
     hipGraphDestroy(graph);
     HIPCHK(c, e);
   }
+  c->dev_steps_ready = true;
+  return AFID_OK;
+}
+
+static int dev_steps_enqueue(afid_ctx *c, int nsteps) {
   for (int n = 0; n < nsteps; n++) HIPCHK(c, hipGraphLaunch(c->step_graph, c->st));
+  return AFID_OK;
+}
+
+static int dev_steps_finish(afid_ctx *c, double *dt_last, int *errorcode, int *steps_done) {
+  DevStep h;
+  HIPCHK(c, hipMemcpyAsync(&h, c->dstep, sizeof(DevStep), hipMemcpyDeviceToHost, c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));
-  HIPCHK(c, hipMemcpy(&h, c->dstep, sizeof(DevStep), hipMemcpyDeviceToHost));
   c->time = h.time;
   if (dt_last) *dt_last = h.dt;
   if (errorcode) *errorcode = h.errorcode;
   if (steps_done) *steps_done = h.steps_done;
   return h.pivot ? AFID_NUM_PIVOT : AFID_OK;
+}
+
+int afid_steps(afid_ctx *c, int ntime0, int nsteps, int idtv, double dt, double dtmin, double dtmax,
+               double limitcfl, double *dt_last, int *errorcode, int *steps_done) {
+  if (!c || nsteps < 0 || (!idtv && dt <= 0)) return AFID_ERR_ARG;
+  if (c->singular_mode) return AFID_NUM_SINGULAR;
+  if (!c->split) {   // the fused update swaps buffers per substep: step from the host
+    int err = 0, done = 0;
+    double dtu = dt;
+    for (int n = 0; n < nsteps && err == 0; n++, done++) {
+      int rc = afid_step(c, ntime0 + n, idtv, dt, dtmin, dtmax, limitcfl, &dtu, &err);
+      if (rc) return rc;
+    }
+    if (dt_last) *dt_last = dtu;
+    if (errorcode) *errorcode = err;
+    if (steps_done) *steps_done = done;
+    return AFID_OK;
+  }
+  int rc = dev_steps_prepare(c, ntime0, idtv, dt, dtmin, dtmax, limitcfl);
+  if (!rc) rc = dev_steps_enqueue(c, nsteps);
+  if (rc) return rc;
+  return dev_steps_finish(c, dt_last, errorcode, steps_done);
+}
+
+int afid_steps_async(afid_ctx *c, int ntime0, int nsteps, int idtv, double dt, double dtmin, double dtmax,
+                     double limitcfl) {
+  if (!c || nsteps < 0 || (!idtv && dt <= 0)) return AFID_ERR_ARG;
+  if (c->singular_mode) return AFID_NUM_SINGULAR;
+  if (!c->split) return AFID_ERR_STATE;
+  int rc = dev_steps_prepare(c, ntime0, idtv, dt, dtmin, dtmax, limitcfl);
+  return rc ? rc : dev_steps_enqueue(c, nsteps);
+}
+
+int afid_steps_more(afid_ctx *c, int nsteps) {
+  if (!c || nsteps < 0) return AFID_ERR_ARG;
+  if (!c->dev_steps_ready) return AFID_ERR_STATE;
+  return dev_steps_enqueue(c, nsteps);
+}
+
+int afid_steps_query(afid_ctx *c, int *busy) {
+  if (!c || !busy) return AFID_ERR_ARG;
+  hipError_t e = hipStreamQuery(c->st);
+  if (e == hipErrorNotReady) {
+    *busy = 1;
+    return AFID_OK;
+  }
+  *busy = 0;
+  HIPCHK(c, e);
+  return AFID_OK;
+}
+
+int afid_steps_wait(afid_ctx *c, double *dt_last, int *errorcode, int *steps_done) {
+  if (!c) return AFID_ERR_ARG;
+  if (!c->dev_steps_ready) return AFID_ERR_STATE;
+  return dev_steps_finish(c, dt_last, errorcode, steps_done);
+}
+
+int afid_batch_steps(afid_ctx *const *cells, int ncells, int nsteps) {
+  if (!cells || ncells < 0 || nsteps < 0) return AFID_ERR_ARG;
+  for (int q = 0; q < ncells; q++) {
+    if (!cells[q]) return AFID_ERR_ARG;
+    if (!cells[q]->dev_steps_ready) return AFID_ERR_STATE;
+  }
+  // step n of every cell is queued before step n+1 of any, each cell on its own stream
+  for (int n = 0; n < nsteps; n++)
+    for (int q = 0; q < ncells; q++) {
+      int rc = dev_steps_enqueue(cells[q], 1);
+      if (rc) return rc;
+    }
+  return AFID_OK;
+}
+
+int afid_estimate_bytes(const afid_params *p, int64_t *bytes) {
+  if (!p || !bytes || p->nxm < 2 || p->nym < 2 || p->nzm < 2) return AFID_ERR_ARG;
+  const int64_t nx = p->nxm + 1, nxm = p->nxm, nym = p->nym, nzm = p->nzm;
+  const int64_t L = (nx + P - 1) / P, ldx = P * L, ncol = nym * nzm;
+  const int64_t ldm = ((nym / 2 + 1) + 3) / 4 * 4;
+  const int64_t field = 8 * ncol * ldx, fac = 8 * (int64_t)SYS_N * FR_N * ldx;
+  int64_t b = 14 * field;                                   // 10 fields + 4 b' (or 4 ping-pong) arrays
+  b += 8 * ncol;                                            // stale row
+  b += nxm * nzm * ldm * (16 + 8);                          // spectrum and x-solve scratch
+  b += 8 * (int64_t)KC_N * ldx + 8 * (2 * nx + 7 * nxm + nym + nzm);   // coefficient rows
+  b += 4 * fac + 3 * fac + 8 * (int64_t)SYS_N * 6 * nx + 8 * 7 * nx + (int64_t)sizeof(DevStep);
+  b += 20 * (nym + nzm) + 16 * nym;                         // FFT tables
+  b += 8 * (reduce_partial_blocks() * 16 + 4 * nxm + 32);   // reduction scratch
+  b += 8 * nx * (nym + 2) * (nzm + 2);                      // staging of uploads and downloads
+  Geom gs{};
+  gs.ldx = (int)ldx;
+  b += 8 * (STATS_N * nxm + (int64_t)stats_partial_doubles(gs));  // statistics
+  b += (4 + 8 * 4 * ncol) * 9;                              // nine slab planes of four fields
+  *bytes = b;
+  return AFID_OK;
 }
 
 int afid_get_time(afid_ctx *c, double *t) {
@@ -1327,6 +1433,7 @@ int afid_calc_stats(afid_ctx *c) {
     HIPCHK(c, hipMalloc(&c->stat_acc, sizeof(double) * STATS_N * c->g.nxm));
     HIPCHK(c, hipMemsetAsync(c->stat_acc, 0, sizeof(double) * STATS_N * c->g.nxm, c->st));
     HIPCHK(c, hipMalloc(&c->stat_partial, sizeof(double) * stats_partial_doubles(c->g)));
+    c->bytes_held += sizeof(double) * (STATS_N * c->g.nxm + stats_partial_doubles(c->g));
   }
   ReduceArgs a = reduce_args(c);
   a.partial = c->stat_partial;
@@ -1368,12 +1475,14 @@ int afid_get_slabs(afid_ctx *c, const int *kslab, int nslab, double *out) {
   if (nslab > c->slab_n) {
     hipFree(c->d_kslab);
     hipFree(c->slab_buf);
+    c->bytes_held -= (sizeof(int) + sizeof(double) * 4 * (size_t)c->g.ncol) * c->slab_n;
     c->d_kslab = nullptr;
     c->slab_buf = nullptr;
     c->slab_n = 0;
     HIPCHK(c, hipMalloc(&c->d_kslab, sizeof(int) * nslab));
     HIPCHK(c, hipMalloc(&c->slab_buf, sizeof(double) * 4 * (size_t)nslab * c->g.ncol));
     c->slab_n = nslab;
+    c->bytes_held += (sizeof(int) + sizeof(double) * 4 * (size_t)c->g.ncol) * nslab;
   }
   HIPCHK(c, hipStreamSynchronize(c->st));
   HIPCHK(c, hipMemcpy(c->d_kslab, kslab, sizeof(int) * nslab, hipMemcpyHostToDevice));

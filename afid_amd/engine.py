@@ -74,6 +74,12 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "afid_time_marcher": ([vp, d], i),
         "afid_step": ([vp, i, i, d, d, d, d, pd, ctypes.POINTER(i)], i),
         "afid_steps": ([vp, i, i, i, d, d, d, d, pd, ctypes.POINTER(i), ctypes.POINTER(i)], i),
+        "afid_steps_async": ([vp, i, i, i, d, d, d, d], i),
+        "afid_steps_more": ([vp, i], i),
+        "afid_steps_query": ([vp, ctypes.POINTER(i)], i),
+        "afid_steps_wait": ([vp, pd, ctypes.POINTER(i), ctypes.POINTER(i)], i),
+        "afid_batch_steps": ([ctypes.POINTER(vp), i, i], i),
+        "afid_estimate_bytes": ([ctypes.c_void_p, ctypes.POINTER(i64)], i),
         "afid_get_time": ([vp, pd], i),
         "afid_set_time": ([vp, d], i),
         "afid_get_stale": ([vp, pd], i),
@@ -209,6 +215,26 @@ def _dp(a: np.ndarray):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
+def estimate_bytes(params: "CellParams") -> int:
+    """Device bytes a cell of these parameters holds at most."""
+    L = load_library()
+    cp = params.to_c()
+    out = ctypes.c_int64()
+    rc = L.afid_estimate_bytes(ctypes.byref(cp), ctypes.byref(out))
+    if rc:
+        raise AfidError(f"estimate_bytes: {L.afid_strerror(rc).decode()} ({rc})")
+    return out.value
+
+
+def batch_steps(cells, nsteps: int):
+    """Queues nsteps more steps on every cell, interleaved (each had steps_async)."""
+    L = load_library()
+    hs = (ctypes.c_void_p * len(cells))(*[c.h.value for c in cells])
+    rc = L.afid_batch_steps(hs, len(cells), nsteps)
+    if rc:
+        raise AfidError(f"batch_steps: {L.afid_strerror(rc).decode()} ({rc})")
+
+
 class Cell:
     """One Rayleigh-Benard cell resident on the GPU."""
 
@@ -221,6 +247,12 @@ class Cell:
         dims = (ctypes.c_int64 * 6)()
         self._check(self.L.afid_get_dims(self.h, dims), "afid_get_dims")
         self.nx, self.nxm, self.nym, self.nzm, self.ldx, self.device_bytes = list(dims)
+
+    def held_bytes(self) -> int:
+        """Device bytes the context holds now (buffers allocated so far)."""
+        dims = (ctypes.c_int64 * 6)()
+        self._check(self.L.afid_get_dims(self.h, dims), "afid_get_dims")
+        return int(dims[5])
 
     # ---- plumbing ---------------------------------------------------------------
     def _check(self, rc: int, what: str):
@@ -329,6 +361,28 @@ class Cell:
         err, done = ctypes.c_int(), ctypes.c_int()
         self._check(self.L.afid_steps(self.h, ntime0, nsteps, idtv, dt, dtmin, dtmax, limitcfl, ctypes.byref(used),
                                       ctypes.byref(err), ctypes.byref(done)), "steps")
+        return err.value, used.value, done.value
+
+    def steps_async(self, ntime0: int, nsteps: int, idtv: int, dt: float, dtmin: float = 1e-8,
+                    dtmax: float = 1e-2, limitcfl: float = 1.2):
+        """Queues nsteps device steps on this cell's stream and returns at once."""
+        self._check(self.L.afid_steps_async(self.h, ntime0, nsteps, idtv, dt, dtmin, dtmax, limitcfl),
+                    "steps_async")
+
+    def steps_more(self, nsteps: int):
+        self._check(self.L.afid_steps_more(self.h, nsteps), "steps_more")
+
+    def busy(self) -> bool:
+        b = ctypes.c_int()
+        self._check(self.L.afid_steps_query(self.h, ctypes.byref(b)), "steps_query")
+        return bool(b.value)
+
+    def steps_wait(self):
+        """(errorcode, dt of the last step, steps run since steps_async)."""
+        used = ctypes.c_double()
+        err, done = ctypes.c_int(), ctypes.c_int()
+        self._check(self.L.afid_steps_wait(self.h, ctypes.byref(used), ctypes.byref(err), ctypes.byref(done)),
+                    "steps_wait")
         return err.value, used.value, done.value
 
     @property

@@ -145,6 +145,30 @@ int afid_step(afid_ctx *ctx, int ntime, int idtv, double dt, double dtmin, doubl
  * interchange. */
 int afid_steps(afid_ctx *ctx, int ntime0, int nsteps, int idtv, double dt, double dtmin, double dtmax,
                double limitcfl, double *dt_last, int *errorcode, int *steps_done);
+
+/* ---- concurrent cells (batched sweeps, serving) --------------------------------
+ * Each context has its own stream and its own captured step graph, so the
+ * steps of several cells overlap on the GPU.  No counterpart in the
+ * reference, which runs one cell per job. */
+/* afid_steps without the final wait: sets the device step state as afid_steps
+ * does, queues nsteps steps and returns.  AFID_ERR_STATE with AFID_UPDATE=fused. */
+int afid_steps_async(afid_ctx *ctx, int ntime0, int nsteps, int idtv, double dt, double dtmin, double dtmax,
+                     double limitcfl);
+/* Queues nsteps more steps that continue the device step state (time, ntime,
+ * dt, halt). */
+int afid_steps_more(afid_ctx *ctx, int nsteps);
+/* *busy = 1 while queued work of this context has not finished. */
+int afid_steps_query(afid_ctx *ctx, int *busy);
+/* Waits for the queued steps; results as afid_steps (steps_done counts from
+ * the last afid_steps_async). */
+int afid_steps_wait(afid_ctx *ctx, double *dt_last, int *errorcode, int *steps_done);
+/* Queues nsteps more steps on each of ncells contexts, step n of every cell
+ * before step n+1 of any; each cell must have had afid_steps_async. */
+int afid_batch_steps(afid_ctx *const *cells, int ncells, int nsteps);
+/* Device bytes a context for p holds at most (fields, spectra, factor rows,
+ * device step state, staging, statistics, nine slab planes); for admission
+ * by memory budget. */
+int afid_estimate_bytes(const afid_params *p, int64_t *bytes);
 int afid_get_time(afid_ctx *ctx, double *time);
 int afid_set_time(afid_ctx *ctx, double time);      /* time of a continuation (ReadFlowField.F90:46) */
 /* The stale row rhs(1,j,i) the x-solves read (SolveImpEqnUpdate_X.F90:44):
