@@ -34,7 +34,11 @@ FIELD_IDS = {"vx": AFID_VX, "vy": AFID_VY, "vz": AFID_VZ, "temp": AFID_TEMP, "pr
 
 
 class AfidError(RuntimeError):
-    pass
+    """A nonzero return code of the library; .code holds it (0 when none applies)."""
+
+    def __init__(self, msg: str, code: int = 0):
+        super().__init__(msg)
+        self.code = code
 
 
 class _Params(ctypes.Structure):
@@ -159,7 +163,7 @@ def slab_levels(xm: np.ndarray, xslab: np.ndarray) -> np.ndarray:
     out = (ctypes.c_int * 9)()
     rc = load_io_library().afid_slab_levels(_dp(xm), len(xm), _dp(xs), out)
     if rc:
-        raise AfidError(f"slab_levels: {rc}")
+        raise AfidError(f"slab_levels: {rc}", rc)
     return np.array(out[:], dtype=np.int32)
 
 
@@ -169,7 +173,7 @@ def h5_write_field(path: str, dset: str, a: np.ndarray) -> None:
     n3, n2, n1 = a.shape
     rc = load_io_library().afid_h5_write_field(path.encode(), dset.encode(), _dp(a), n1, n2, n3)
     if rc:
-        raise AfidError(f"h5_write_field: {load_library().afid_strerror(rc).decode()} ({rc})")
+        raise AfidError(f"h5_write_field: {load_library().afid_strerror(rc).decode()} ({rc})", rc)
 
 
 def h5_read_field(path: str, dset: str, shape: tuple) -> np.ndarray:
@@ -178,7 +182,7 @@ def h5_read_field(path: str, dset: str, shape: tuple) -> np.ndarray:
     n3, n2, n1 = shape
     rc = load_io_library().afid_h5_read_field(path.encode(), dset.encode(), _dp(a), n1, n2, n3)
     if rc:
-        raise AfidError(f"h5_read_field: {load_library().afid_strerror(rc).decode()} ({rc})")
+        raise AfidError(f"h5_read_field: {load_library().afid_strerror(rc).decode()} ({rc})", rc)
     return a
 
 
@@ -222,7 +226,7 @@ def estimate_bytes(params: "CellParams") -> int:
     out = ctypes.c_int64()
     rc = L.afid_estimate_bytes(ctypes.byref(cp), ctypes.byref(out))
     if rc:
-        raise AfidError(f"estimate_bytes: {L.afid_strerror(rc).decode()} ({rc})")
+        raise AfidError(f"estimate_bytes: {L.afid_strerror(rc).decode()} ({rc})", rc)
     return out.value
 
 
@@ -232,7 +236,7 @@ def batch_steps(cells, nsteps: int):
     hs = (ctypes.c_void_p * len(cells))(*[c.h.value for c in cells])
     rc = L.afid_batch_steps(hs, len(cells), nsteps)
     if rc:
-        raise AfidError(f"batch_steps: {L.afid_strerror(rc).decode()} ({rc})")
+        raise AfidError(f"batch_steps: {L.afid_strerror(rc).decode()} ({rc})", rc)
 
 
 class Cell:
@@ -259,7 +263,7 @@ class Cell:
         if rc != 0:
             msg = self.L.afid_strerror(rc).decode()
             hip = self.L.afid_last_hip_error(self.h) if self.h else 0
-            raise AfidError(f"{what}: {msg} (code {rc}, hip {hip})")
+            raise AfidError(f"{what}: {msg} (code {rc}, hip {hip})", rc)
 
     def close(self):
         if self.h:

@@ -25,7 +25,7 @@ from typing import Callable, Dict, List, Optional
 
 import numpy as np
 
-from .engine import Cell, CellParams, estimate_bytes
+from .engine import AfidError, Cell, CellParams, estimate_bytes
 from .sweep import STEP_CTL, CellSpec
 
 
@@ -74,6 +74,7 @@ class Completion:
     errorcode: int
     max_divergence: float
     cell_updates: int
+    refused: str = ""       # why the library refused the cell (the reference would abort the job)
 
     @property
     def latency(self) -> float:
@@ -133,13 +134,24 @@ class Server:
             while waiting and used + self.estimate(waiting[0].spec.params) <= self.budget:
                 r = waiting.popleft()
                 nb = self.estimate(r.spec.params)
-                cell = self.make_cell(r.spec.params)
-                cell.init_linear_noise(r.spec.seed, 1e-3)
-                first = min(self.chunk, r.spec.steps)
-                cell.steps_async(0, first, ctl["idtv"], ctl["dt"], ctl["dtmin"], ctl["dtmax"], ctl["limitcfl"])
+                progressed = True
+                cell = None
+                try:
+                    cell = self.make_cell(r.spec.params)
+                    cell.init_linear_noise(r.spec.seed, 1e-3)
+                    first = min(self.chunk, r.spec.steps)
+                    cell.steps_async(0, first, ctl["idtv"], ctl["dt"], ctl["dtmin"], ctl["dtmax"],
+                                     ctl["limitcfl"])
+                except AfidError as e:
+                    # e.g. the singular zero pressure mode, where the reference aborts
+                    # (SolvePressureCorrection.F90:130-134): the request ends unrun
+                    if cell is not None:
+                        cell.close()
+                    t = self.clock() - t0
+                    done.append(Completion(r.rid, r.arrival, t, t, 0, e.code, 0.0, 0, str(e)))
+                    continue
                 active[r.rid] = _Active(r, cell, first, self.clock() - t0, nb)
                 used += nb
-                progressed = True
             self.peak_bytes = max(self.peak_bytes, used)
             self.peak_active = max(self.peak_active, len(active))
             for rid in list(active):
@@ -148,7 +160,7 @@ class Server:
                     continue
                 progressed = True
                 if a.queued < a.req.spec.steps:
-                    # two chunks ahead, so the stream stays fed between polls
+                    # later chunks are twice the first: fewer idle gaps between polls
                     n = min(2 * self.chunk, a.req.spec.steps - a.queued)
                     a.cell.steps_more(n)
                     a.queued += n
@@ -171,11 +183,14 @@ class Server:
 
 
 def summarize(done: List[Completion], makespan: Optional[float] = None) -> dict:
-    lat = np.array([c.latency for c in done])
+    """Latency percentiles over the requests that ran; refused ones are counted apart."""
+    ran = [c for c in done if not c.refused]
+    lat = np.array([c.latency for c in ran])
     span = makespan if makespan is not None else max(c.finished for c in done)
-    updates = sum(c.cell_updates for c in done)
-    return {"requests": len(done), "p50_latency_s": float(np.percentile(lat, 50)),
-            "p99_latency_s": float(np.percentile(lat, 99)), "makespan_s": span,
+    updates = sum(c.cell_updates for c in ran)
+    return {"requests": len(done), "refused": len(done) - len(ran),
+            "p50_latency_s": float(np.percentile(lat, 50)) if len(ran) else None,
+            "p99_latency_s": float(np.percentile(lat, 99)) if len(ran) else None, "makespan_s": span,
             "Mcell_updates_per_s": updates / span / 1e6,
-            "max_divergence": max(c.max_divergence for c in done),
-            "errorcodes": sorted({c.errorcode for c in done})}
+            "max_divergence": max((c.max_divergence for c in ran), default=0.0),
+            "errorcodes": sorted({c.errorcode for c in ran})}

@@ -115,6 +115,54 @@ def copy_calibration(mib: int) -> None:
     hip.hipFree(b)
 
 
+def bench_sweep(args):
+    """Batched sweep: all cells resident, steps interleaved; beside it the same
+    cells one at a time through the plain host-driven path (afid_step)."""
+    from afid_amd.sweep import run_batched, run_serial, sweep_specs
+    specs = sweep_specs(args.sweep_cells, args.sweep_steps, seed=0)
+    print(f"sweep: {len(specs)} cells x {args.sweep_steps} steps, batched", file=sys.stderr, flush=True)
+    b = run_batched(specs)
+    print("sweep: serial host-driven", file=sys.stderr, flush=True)
+    h = run_serial(specs, host_driven=True)
+    out = {
+        "metric": "cells/hour", "value": round(b.cells_per_hour, 1), "unit": "cells/hour", "n_gpus": 1,
+        "steps": args.sweep_steps, "warmup": 0, "ms_per_step": None, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded conduction + noise)",
+        "config": {"workload": f"batched sweep: {len(specs)} cells, nxm in {{32..128}}, ny=nz in {{64..256}}, "
+                               f"Ra 1e5-1e7, Pr in {{0.7,1,4.4}}, aspect 2, istr3=6 str3=6, CFL dt, "
+                               f"{args.sweep_steps} steps each"},
+        "batched": b.as_dict(), "serial_host": h.as_dict(),
+        "speedup_vs_serial_host": round(h.seconds / b.seconds, 3),
+        "max_divergence": max(b.max_divergence, h.max_divergence),
+    }
+    print(json.dumps(out))
+
+
+def bench_serve(args):
+    """Serving: Poisson arrivals (4/s) admitted by a 200 GB device budget."""
+    from afid_amd.serve import Server, serving_requests, summarize
+    reqs = serving_requests(args.requests, lam=4.0, seed=0)
+    print(f"serve: {len(reqs)} requests", file=sys.stderr, flush=True)
+    srv = Server(budget_bytes=200e9, chunk=50)
+    t0 = time.perf_counter()
+    done = srv.run(reqs)
+    span = time.perf_counter() - t0
+    s = summarize(done, span)
+    out = {
+        "metric": "Mcell-updates/s", "value": round(s["Mcell_updates_per_s"], 2), "unit": "Mcell-updates/s",
+        "n_gpus": 1, "steps": sum(r.spec.steps for r in reqs), "warmup": 0, "ms_per_step": None,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded conduction + noise)",
+        "config": {"workload": f"serving: {len(reqs)} requests, Poisson 4/s, nxm U[32,192], ny=nz 2-3-5 smooth "
+                               f"in [64,384], Ra 1e5-1e8, Pr U[0.7,7], steps U[200,2000], 200 GB budget"},
+        "p50_latency_s": round(s["p50_latency_s"], 3), "p99_latency_s": round(s["p99_latency_s"], 3),
+        "refused": s["refused"],
+        "makespan_s": round(span, 3), "peak_active": srv.peak_active, "peak_bytes": srv.peak_bytes,
+        "max_divergence": s["max_divergence"], "errorcodes": s["errorcodes"],
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -126,11 +174,21 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
     ap.add_argument("--profile-steps", type=int, default=2)
+    ap.add_argument("--workload", default="cell", choices=("cell", "sweep", "serve"),
+                    help="cell: one production cell (default); sweep: 64-cell batched parameter sweep; "
+                         "serve: Poisson request stream")
+    ap.add_argument("--sweep-cells", type=int, default=64)
+    ap.add_argument("--sweep-steps", type=int, default=500)
+    ap.add_argument("--requests", type=int, default=24)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(each cell runs on one device; see DESIGN.md for how the path would shard)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "sweep":
+        return bench_sweep(args)
+    if args.workload == "serve":
+        return bench_serve(args)
 
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)

@@ -209,3 +209,22 @@ def test_estimate_bytes_covers_allocation():
     est = estimate_bytes(spec.params)
     assert held <= est <= 1.02 * held + 65536
     c.close()
+
+
+class SingularCell(FakeCell):
+    def steps_async(self, *a):
+        from afid_amd.engine import AfidError
+        raise AfidError("steps_async: singular pressure tridiagonal", 2)
+
+
+def test_server_records_refused_request():
+    def make(p):
+        return SingularCell(p) if p.nxm == 33 else FakeCell(p)
+    FakeCell.live = 0
+    reqs = [_req(0, 0.0, 40, 20), _req(1, 0.0, 33, 20), _req(2, 0.001, 50, 20)]
+    done = Server(budget_bytes=1e9, chunk=8, make_cell=make, estimate=lambda p: 1000).run(reqs)
+    by = {c.rid: c for c in done}
+    assert by[1].refused and by[1].errorcode == 2 and by[1].steps_done == 0
+    assert by[0].steps_done == by[2].steps_done == 20 and FakeCell.live == 0
+    s = summarize(done)
+    assert s["requests"] == 3 and s["refused"] == 1
