@@ -82,6 +82,7 @@ struct afid_ctx {
   size_t stage_bytes = 0;
   size_t bytes_held = 0;
   bool dev_steps_ready = false;   // device step state set and the step graph captured
+  bool mixed = false;             // AFID_PREC_MIXED: FP32 histories and FFT spectrum
   DevPlan dpy, dpz;
   RealPlan py{};
   FFTPlan pz{};
@@ -525,6 +526,7 @@ PoissonArgs poisson_args(afid_ctx *c, int from_velocity, int ns = 0, bool dev = 
   a.py = c->py;
   a.pz = c->pz;
   a.from_velocity = from_velocity;
+  a.spec32 = c->mixed;
   return a;
 }
 
@@ -633,7 +635,7 @@ int afid_create(afid_ctx **out, const afid_params *p, const afid_metrics *m) {
   if (!out || !p) return AFID_ERR_ARG;
   *out = nullptr;
   if (p->nxm < 2 || p->nym < 2 || p->nzm < 2 || p->alx3 <= 0 || p->ylen <= 0 || p->zlen <= 0 ||
-      p->ray <= 0 || p->pra <= 0)
+      p->ray <= 0 || p->pra <= 0 || (p->precision != AFID_PREC_FP64 && p->precision != AFID_PREC_MIXED))
     return AFID_ERR_ARG;
   afid_ctx *c = new afid_ctx();
   c->p = *p;
@@ -706,6 +708,8 @@ int afid_create(afid_ctx **out, const afid_params *p, const afid_metrics *m) {
   for (int q : {AFID_VX, AFID_VY, AFID_VZ, AFID_TEMP, AFID_PR, AFID_RUX, AFID_RUY, AFID_RUZ, AFID_RUTEMP, AFID_DPH})
     TRY(alloc_field(c, &c->f[q]));
   if (const char *e = getenv("AFID_UPDATE")) c->split = strcmp(e, "fused") != 0;
+  c->mixed = p->precision == AFID_PREC_MIXED;
+  if (c->mixed && !c->split) TRY(AFID_ERR_ARG);   // the mixed mode runs on the split update
   if (c->split)
     for (int q = 0; q < 4; q++) TRY(alloc_field(c, &c->bp[q]));
   else
@@ -853,6 +857,10 @@ static int host_extent(afid_ctx *c, int field, int ld1, int ld2, int j0, int i0,
   return AFID_OK;
 }
 
+static bool is_history(int field) {
+  return field == AFID_RUX || field == AFID_RUY || field == AFID_RUZ || field == AFID_RUTEMP;
+}
+
 int afid_upload(afid_ctx *c, int field, const double *host, int ld1, int ld2, int j0, int i0) {
   if (!c || !host || field < 0 || field >= AFID_NFIELDS) return AFID_ERR_ARG;
   size_t n;
@@ -863,8 +871,13 @@ int afid_upload(afid_ctx *c, int field, const double *host, int ld1, int ld2, in
   if ((rc = ensure_stage(c, n * sizeof(double)))) return rc;
   HIPCHK(c, hipStreamSynchronize(c->st));
   HIPCHK(c, hipMemcpy(c->stage, host, n * sizeof(double), hipMemcpyHostToDevice));
-  if ((rc = kerr(c, launch_gather_in(c->stage, c->f[field], c->g, field_rows(c, field), ld1, ld2, j0, i0, c->st))))
+  if (c->mixed && is_history(field)) {   // FP64 layout into b' scratch, then FP32 into the history
+    if ((rc = kerr(c, launch_gather_in(c->stage, c->bp[0], c->g, c->g.nx, ld1, ld2, j0, i0, c->st)))) return rc;
+    if ((rc = kerr(c, launch_convert(c->bp[0], c->f[field], c->g.ncol * c->g.ldx, 1, c->st)))) return rc;
+  } else if ((rc = kerr(c, launch_gather_in(c->stage, c->f[field], c->g, field_rows(c, field), ld1, ld2, j0, i0,
+                                            c->st)))) {
     return rc;
+  }
   if (field == AFID_RHS) {  // rhs(1,j,i) is the stale row shared by the x-solves
     // gather row k=0 into the stale plane: reuse scatter of a 1-row view on the host side
     std::vector<double> plane((size_t)c->g.ncol);
@@ -886,8 +899,12 @@ int afid_download(afid_ctx *c, int field, double *host, int ld1, int ld2, int j0
   if ((rc = ensure_stage(c, n * sizeof(double)))) return rc;
   const int halo = (j0 == 0 && i0 == 0);
   HIPCHK(c, hipMemsetAsync(c->stage, 0, n * sizeof(double), c->st));
-  if ((rc = kerr(c, launch_scatter_out(c->f[field], c->stage, c->g, field_rows(c, field), ld1, ld2, j0, i0, halo,
-                                       c->st))))
+  const double *src = c->f[field];
+  if (c->mixed && is_history(field)) {   // FP32 history widened into b' scratch first
+    if ((rc = kerr(c, launch_convert(c->f[field], c->bp[0], c->g.ncol * c->g.ldx, 0, c->st)))) return rc;
+    src = c->bp[0];
+  }
+  if ((rc = kerr(c, launch_scatter_out(src, c->stage, c->g, field_rows(c, field), ld1, ld2, j0, i0, halo, c->st))))
     return rc;
   HIPCHK(c, hipStreamSynchronize(c->st));
   HIPCHK(c, hipMemcpy(host, c->stage, n * sizeof(double), hipMemcpyDeviceToHost));
@@ -958,6 +975,7 @@ static int implicit_update(afid_ctx *c, int which) {
   static const int ru_of[4] = {AFID_RUX, AFID_RUY, AFID_RUZ, AFID_RUTEMP};
   static const int q_of[4] = {AFID_VX, AFID_VY, AFID_VZ, AFID_TEMP};
   static const char *names[4] = {"implicit_vx", "implicit_vy", "implicit_vz", "implicit_temp"};
+  if (c->mixed) return AFID_ERR_STATE;   // these read and write the histories in FP64
   int rc;
   if (!c->f[expl_of[which]]) return AFID_ERR_STATE;
   if ((rc = alloc_field(c, &c->f[AFID_RHS]))) return rc;
@@ -1070,6 +1088,7 @@ static int enqueue_substep(afid_ctx *c, int ns, bool dev) {
   a.sc.use_ru = c->rom[ns] != 0.0;
   a.ds = dev ? c->dstep : nullptr;
   a.ns = ns;
+  a.h32 = c->mixed;
   if (c->split) {
     {
       Timed t(c, "stencil_rhs");
@@ -1102,13 +1121,22 @@ int afid_substep(afid_ctx *c, int ns, double dt) {
   return enqueue_substep(c, ns, false);
 }
 
+// Mixed precision: the divergence the FP32 pressure solve of the last
+// substep left behind, computed in FP64, is solved for and corrected once more
+// (velocity and pressure, with the last substep's scalars).
+static int refine_projection(afid_ctx *c, int ns, bool dev) {
+  int rc = run_poisson(c, 1, ns, dev);
+  return rc ? rc : correct(c, 3, "refine_correct", ns, dev);
+}
+
 int afid_time_marcher(afid_ctx *c, double dt) {
   if (!c) return AFID_ERR_ARG;
-  for (int ns = 1; ns <= c->p.nsst && ns <= 3; ns++) {
+  int ns = 1;
+  for (; ns <= c->p.nsst && ns <= 3; ns++) {
     int rc = afid_substep(c, ns, dt);
     if (rc) return rc;
   }
-  return AFID_OK;
+  return c->mixed ? refine_projection(c, ns - 1, false) : AFID_OK;
 }
 
 int afid_calc_max_cfl(afid_ctx *c, double *cflm) {
@@ -1195,6 +1223,7 @@ static int enqueue_step_device(afid_ctx *c) {
   if ((rc = kerr(c, launch_factor(fa, c->st)))) return rc;
   for (int ns = 1; ns <= c->kconst.nsst; ns++)
     if ((rc = enqueue_substep(c, ns, true))) return rc;
+  if (c->mixed && (rc = refine_projection(c, c->kconst.nsst, true))) return rc;
   return kerr(c, launch_step_end(c->dstep, c->st));
 }
 

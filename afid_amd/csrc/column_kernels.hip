@@ -268,6 +268,18 @@ template <typename OffT>
 __device__ __forceinline__ void stf(double *base, OffT boff, double v) {
   *reinterpret_cast<double *>(reinterpret_cast<char *>(base) + boff) = v;
 }
+// History arrays (rux..rutemp): FP64, or FP32 in the mixed-precision mode (H32),
+// where element e sits at byte 4e, half the FP64 byte offset.
+template <bool H32, typename OffT>
+__device__ __forceinline__ double ldh(const double *base, OffT boff) {
+  if (H32) return (double)*reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + (boff >> 1));
+  return ldf(base, boff);
+}
+template <bool H32, typename OffT>
+__device__ __forceinline__ void sth(double *base, OffT boff, double v) {
+  if (H32) *reinterpret_cast<float *>(reinterpret_cast<char *>(base) + (boff >> 1)) = (float)v;
+  else stf(base, boff, v);
+}
 
 template <typename OffT>
 struct ColB {
@@ -584,7 +596,7 @@ __device__ __forceinline__ Plane load_plane(const double *__restrict__ vx, const
 // DEV: the substep scalars come from the device step state (a.ds) rather
 // than from the argument block; two instantiations keep each one's scalar
 // loads unconditional.
-template <typename OffT, bool USE_RU, bool DEV>
+template <typename OffT, bool USE_RU, bool DEV, bool H32>
 __global__ void __launch_bounds__(STENCIL_WG)
     k_stencil_rhs(FusedArgs a, const double *__restrict__ vx, const double *__restrict__ vy,
                   const double *__restrict__ vz, const double *__restrict__ T, const double *__restrict__ pr,
@@ -645,8 +657,8 @@ __global__ void __launch_bounds__(STENCIL_WG)
     const double vxjm = ldf(vx, CJM + sb), vxjmp = ldf(vx, CJM + spb), vyjm = ldf(vy, CJM + sb);
     const double vzjm = ldf(vz, CJM + sb), tjm = ldf(T, CJM + sb), prjm = ldf(pr, CJM + sb);
     const double vzjmip = ldf(vz, colb(jm, ip) + sb);
-    const double rxo = USE_RU ? ldf(rux, C + sb) : 0.0, ryo = USE_RU ? ldf(ruy, C + sb) : 0.0;
-    const double rzo = USE_RU ? ldf(ruz, C + sb) : 0.0, rto = USE_RU ? ldf(rut, C + sb) : 0.0;
+    const double rxo = USE_RU ? ldh<H32>(rux, C + sb) : 0.0, ryo = USE_RU ? ldh<H32>(ruy, C + sb) : 0.0;
+    const double rzo = USE_RU ? ldh<H32>(ruz, C + sb) : 0.0, rto = USE_RU ? ldh<H32>(rut, C + sb) : 0.0;
     const double st0 = stalep[(int64_t)ic * g.nym + jc];   // uniform address
     const double stale0 = k == 0 ? st0 : 0.0;
     const double vxm = cur.vxm, vxc = cur.vxc, vxp = cur.vxp, vym = cur.vym, vyc = cur.vyc, vyp = cur.vyp;
@@ -669,7 +681,7 @@ __global__ void __launch_bounds__(STENCIL_WG)
       double rhs = (c.ga * q + r + c.alre * dxxvx - dxp) * c.dt;
       double vin = rhs * cfv(SC_XSC) + ((cfv(SC_XDL) * vxm + vxc) + cfv(SC_XDU) * vxp);
       double vbd = stale0 + vxc;
-      stf(rux, C + sb, inx ? q : 0.0);
+      sth<H32>(rux, C + sb, inx ? q : 0.0);
       stf(bx, C + sb, inx ? vin : vbd);
     }
     // ---- vy (ExplicitTermsVY.F90:52-98, ImplicitAndUpdateVY.F90:52-68); kmv/kpv clamps
@@ -687,7 +699,7 @@ __global__ void __launch_bounds__(STENCIL_WG)
       double r = USE_RU ? c.ro * ryo : 0.0;
       double rhs = (c.ga * h + r + c.alre * dxxvy - dyp) * c.dt;
       double vin = rhs * cfv(SC_YSC) + ((cfv(SC_YDL) * vym + vyc) + cfv(SC_YDU) * vyp);
-      stf(ruy, C + sb, iny ? h : 0.0);
+      sth<H32>(ruy, C + sb, iny ? h : 0.0);
       stf(by, C + sb, vin);   // row nx-1 is outside the nxm system and never read
     }
     // ---- vz (ExplicitTermsVZ.F90:56-101, ImplicitAndUpdateVZ.F90:48-64)
@@ -705,7 +717,7 @@ __global__ void __launch_bounds__(STENCIL_WG)
       double r = USE_RU ? c.ro * rzo : 0.0;
       double rhs = (c.ga * h + r + c.alre * dxxvz - dzp) * c.dt;
       double vin = rhs * cfv(SC_YSC) + ((cfv(SC_YDL) * vzm + vzc) + cfv(SC_YDU) * vzp);
-      stf(ruz, C + sb, iny ? h : 0.0);
+      sth<H32>(ruz, C + sb, iny ? h : 0.0);
       stf(bz, C + sb, vin);
     }
     // ---- temperature (ExplicitTermsTemp.F90:52-93, ImplicitAndUpdateTemp.F90:41-54)
@@ -720,7 +732,7 @@ __global__ void __launch_bounds__(STENCIL_WG)
       double r = USE_RU ? c.ro * rto : 0.0;
       double rhs = (c.ga * h + r + c.alpec * dxxt) * c.dt;
       double vin = rhs * cfv(SC_TSC) + ((cfv(SC_TDL) * tm + tc) + cfv(SC_TDU) * tp);
-      stf(rut, C + sb, inx ? h : 0.0);
+      sth<H32>(rut, C + sb, inx ? h : 0.0);
       stf(bt, C + sb, inx ? vin : tc);   // row 1 gets the vz increment added by the solve pass
     }
     // march: plane i becomes i-1, plane i+1 becomes i
@@ -1093,15 +1105,14 @@ hipError_t column_configure_lds(size_t bytes) {
   if (e != hipSuccess) return e;
   AFID_LM_LIST(SETB)
 #undef SETB
-#define SETC(O, R)                                                                                       \
-  e = hipFuncSetAttribute((const void *)k_stencil_rhs<O, R, true>, hipFuncAttributeMaxDynamicSharedMemorySize, \
-                          (int)bytes);                                                                   \
-  if (e != hipSuccess) return e;                                                                         \
-  e = hipFuncSetAttribute((const void *)k_stencil_rhs<O, R, false>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+#define SETC1(O, R, D, H)                                                                                \
+  e = hipFuncSetAttribute((const void *)k_stencil_rhs<O, R, D, H>, hipFuncAttributeMaxDynamicSharedMemorySize, \
                           (int)bytes);                                                                   \
   if (e != hipSuccess) return e;
+#define SETC(O, R) SETC1(O, R, true, false) SETC1(O, R, false, false) SETC1(O, R, true, true) SETC1(O, R, false, true)
   SETC(uint32_t, true) SETC(uint32_t, false) SETC(uint64_t, true) SETC(uint64_t, false)
 #undef SETC
+#undef SETC1
   return hipSuccess;
 }
 
@@ -1143,15 +1154,15 @@ hipError_t launch_stencil_rhs(const FusedArgs &a, hipStream_t st) {
   const dim3 grid((unsigned)(((ntile + 7) / 8) * 8));
   const size_t shm = stencil_lds(a.g);
   const bool small = (uint64_t)n * sizeof(double) < (1ull << 32);
-  const bool ru = a.sc.use_ru != 0, dev = a.ds != nullptr;
-#define SR1(O, R, D)                                                                                     \
-  if (std::is_same<O, uint32_t>::value == small && R == ru && D == dev) {                              \
-    hipLaunchKernelGGL((k_stencil_rhs<O, R, D>), grid, dim3(STENCIL_WG), shm, st, a, a.f.vx, a.f.vy, a.f.vz, \
-                       a.f.temp, a.f.pr, a.rux, a.ruy, a.ruz, a.rut, a.bx, a.by, a.bz, a.bt, a.stale, a.kc, \
-                       a.fac);                                                                         \
+  const bool ru = a.sc.use_ru != 0, dev = a.ds != nullptr, h32 = a.h32 != 0;
+#define SR1(O, R, D, H)                                                                                  \
+  if (std::is_same<O, uint32_t>::value == small && R == ru && D == dev && H == h32) {                  \
+    hipLaunchKernelGGL((k_stencil_rhs<O, R, D, H>), grid, dim3(STENCIL_WG), shm, st, a, a.f.vx, a.f.vy,  \
+                       a.f.vz, a.f.temp, a.f.pr, a.rux, a.ruy, a.ruz, a.rut, a.bx, a.by, a.bz, a.bt, a.stale, \
+                       a.kc, a.fac);                                                                   \
     return hipGetLastError();                                                                          \
   }
-#define SR(O, R) SR1(O, R, true) SR1(O, R, false)
+#define SR(O, R) SR1(O, R, true, false) SR1(O, R, false, false) SR1(O, R, true, true) SR1(O, R, false, true)
   SR(uint32_t, true) SR(uint32_t, false) SR(uint64_t, true) SR(uint64_t, false)
 #undef SR
 #undef SR1

@@ -15,17 +15,35 @@
 
 namespace afid {
 
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ double2 conj2(double2 a) { return make_double2(a.x, -a.y); }
-// multiply by s*i (s = +1/-1)
-__device__ __forceinline__ double2 mul_si(double2 a, double s) { return make_double2(-s * a.y, s * a.x); }
+// Complex arithmetic in FP64 (double2) and, for the mixed-precision mode, FP32
+// (float2); cplx_t<R> names the complex type of real type R.
+template <typename R> struct CplxOf;
+template <> struct CplxOf<double> { using T = double2; };
+template <> struct CplxOf<float> { using T = float2; };
+template <typename R> using cplx_t = typename CplxOf<R>::T;
 
-__device__ __forceinline__ double2 twid(const double2 *tw, int idx, double sign) {
-  double2 t = tw[idx];
+__device__ __forceinline__ double2 mkc(double x, double y) { return make_double2(x, y); }
+__device__ __forceinline__ float2 mkc(float x, float y) { return make_float2(x, y); }
+template <typename C>
+__device__ __forceinline__ C cmul(C a, C b) {
+  return mkc(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return mkc(a.x + b.x, a.y + b.y); }
+template <typename C> __device__ __forceinline__ C csub(C a, C b) { return mkc(a.x - b.x, a.y - b.y); }
+template <typename C> __device__ __forceinline__ C conj2(C a) { return mkc(a.x, -a.y); }
+// multiply by s*i (s = +1/-1)
+template <typename C, typename R> __device__ __forceinline__ C mul_si(C a, R s) { return mkc(-s * a.y, s * a.x); }
+// FP64 table value in the working precision
+template <typename R> __device__ __forceinline__ cplx_t<R> cvt(double2 t) { return mkc((R)t.x, (R)t.y); }
+__device__ __forceinline__ double2 widen(double2 v) { return v; }
+__device__ __forceinline__ double2 widen(float2 v) { return make_double2(v.x, v.y); }
+template <typename C> __device__ __forceinline__ C narrow(double2 v);
+template <> __device__ __forceinline__ double2 narrow<double2>(double2 v) { return v; }
+template <> __device__ __forceinline__ float2 narrow<float2>(double2 v) { return make_float2((float)v.x, (float)v.y); }
+
+template <typename R>
+__device__ __forceinline__ cplx_t<R> twid(const double2 *tw, int idx, R sign) {
+  cplx_t<R> t = cvt<R>(tw[idx]);
   if (sign > 0) t.y = -t.y;
   return t;
 }
@@ -57,8 +75,9 @@ constexpr int TB = 4;   // butterfly iterations per twiddle batch
 // block b and offset u < m, the r points b*m*r + u + v*m (v < r) after
 // twiddling point v by W_{mr}^{uv}.  The twiddles of TB butterflies per lane
 // are loaded together before any of them is computed.
-__device__ __forceinline__ void lds_fft_wave(double2 *lines, int nlines, const FFTPlan &pl, double sign,
-                                             int lane) {
+template <typename R>
+__device__ __forceinline__ void lds_fft_wave(cplx_t<R> *lines, int nlines, const FFTPlan &pl, R sign, int lane) {
+  using C = cplx_t<R>;
   const int n = pl.n;
   const bool sw = (n & 7) == 0;
   const double2 *tw = pl.tw;
@@ -71,7 +90,7 @@ __device__ __forceinline__ void lds_fft_wave(double2 *lines, int nlines, const F
     const unsigned mdiv = pl.mdiv[st];
     const int total = nlines * nb;
     for (int w0 = 0; w0 < total; w0 += 64 * TB) {
-      double2 t1[TB], t2[TB], t3[TB], t4[TB];
+      C t1[TB], t2[TB], t3[TB], t4[TB];
       int lo[TB], e0[TB];
       bool ok[TB];
 #pragma unroll
@@ -94,50 +113,50 @@ __device__ __forceinline__ void lds_fft_wave(double2 *lines, int nlines, const F
 #pragma unroll
       for (int b = 0; b < TB; b++) {
         if (!ok[b]) continue;
-        double2 *line = lines + lo[b];
+        C *line = lines + lo[b];
 #define X(v) line[swz(e0[b] + (v) * m, sw)]
         if (r == 2) {
-          double2 x0 = X(0), x1 = cmul(X(1), t1[b]);
+          C x0 = X(0), x1 = cmul(X(1), t1[b]);
           X(0) = cadd(x0, x1);
           X(1) = csub(x0, x1);
         } else if (r == 4) {
-          double2 x0 = X(0);
-          double2 x1 = cmul(X(1), t1[b]);
-          double2 x2 = cmul(X(2), t2[b]);
-          double2 x3 = cmul(X(3), t3[b]);
-          double2 a0 = cadd(x0, x2), a1 = csub(x0, x2);
-          double2 b0 = cadd(x1, x3), b1 = mul_si(csub(x1, x3), sign);
+          C x0 = X(0);
+          C x1 = cmul(X(1), t1[b]);
+          C x2 = cmul(X(2), t2[b]);
+          C x3 = cmul(X(3), t3[b]);
+          C a0 = cadd(x0, x2), a1 = csub(x0, x2);
+          C b0 = cadd(x1, x3), b1 = mul_si(csub(x1, x3), sign);
           X(0) = cadd(a0, b0);
           X(1) = cadd(a1, b1);
           X(2) = csub(a0, b0);
           X(3) = csub(a1, b1);
         } else if (r == 3) {
-          const double s3 = 0.86602540378443864676372317075294;  // sin(2 pi/3)
-          double2 x0 = X(0);
-          double2 x1 = cmul(X(1), t1[b]);
-          double2 x2 = cmul(X(2), t2[b]);
-          double2 p1 = cadd(x1, x2);
-          double2 p2 = mul_si(csub(x1, x2), sign * s3);
-          double2 q0 = make_double2(x0.x - 0.5 * p1.x, x0.y - 0.5 * p1.y);
+          const R s3 = (R) 0.86602540378443864676372317075294;  // sin(2 pi/3)
+          C x0 = X(0);
+          C x1 = cmul(X(1), t1[b]);
+          C x2 = cmul(X(2), t2[b]);
+          C p1 = cadd(x1, x2);
+          C p2 = mul_si(csub(x1, x2), sign * s3);
+          C q0 = mkc(x0.x - (R)0.5 * p1.x, x0.y - (R)0.5 * p1.y);
           X(0) = cadd(x0, p1);
           X(1) = cadd(q0, p2);
           X(2) = csub(q0, p2);
         } else {  // r == 5
-          const double c1 = 0.30901699437494742410229341718282;   // cos(2 pi/5)
-          const double c2 = -0.80901699437494742410229341718282;  // cos(4 pi/5)
-          const double s1 = 0.95105651629515357211643933337938;   // sin(2 pi/5)
-          const double s2 = 0.58778525229247312916870595463907;   // sin(4 pi/5)
-          double2 x0 = X(0);
-          double2 x1 = cmul(X(1), t1[b]);
-          double2 x2 = cmul(X(2), t2[b]);
-          double2 x3 = cmul(X(3), t3[b]);
-          double2 x4 = cmul(X(4), t4[b]);
-          double2 p1 = cadd(x1, x4), p2 = cadd(x2, x3);
-          double2 p3 = csub(x1, x4), p4 = csub(x2, x3);
-          double2 a1 = make_double2(x0.x + c1 * p1.x + c2 * p2.x, x0.y + c1 * p1.y + c2 * p2.y);
-          double2 a2 = make_double2(x0.x + c2 * p1.x + c1 * p2.x, x0.y + c2 * p1.y + c1 * p2.y);
-          double2 b1 = mul_si(make_double2(s1 * p3.x + s2 * p4.x, s1 * p3.y + s2 * p4.y), sign);
-          double2 b2 = mul_si(make_double2(s2 * p3.x - s1 * p4.x, s2 * p3.y - s1 * p4.y), sign);
+          const R c1 = (R) 0.30901699437494742410229341718282;   // cos(2 pi/5)
+          const R c2 = (R) -0.80901699437494742410229341718282;  // cos(4 pi/5)
+          const R s1 = (R) 0.95105651629515357211643933337938;   // sin(2 pi/5)
+          const R s2 = (R) 0.58778525229247312916870595463907;   // sin(4 pi/5)
+          C x0 = X(0);
+          C x1 = cmul(X(1), t1[b]);
+          C x2 = cmul(X(2), t2[b]);
+          C x3 = cmul(X(3), t3[b]);
+          C x4 = cmul(X(4), t4[b]);
+          C p1 = cadd(x1, x4), p2 = cadd(x2, x3);
+          C p3 = csub(x1, x4), p4 = csub(x2, x3);
+          C a1 = mkc(x0.x + c1 * p1.x + c2 * p2.x, x0.y + c1 * p1.y + c2 * p2.y);
+          C a2 = mkc(x0.x + c2 * p1.x + c1 * p2.x, x0.y + c2 * p1.y + c1 * p2.y);
+          C b1 = mul_si(mkc(s1 * p3.x + s2 * p4.x, s1 * p3.y + s2 * p4.y), sign);
+          C b2 = mul_si(mkc(s2 * p3.x - s1 * p4.x, s2 * p3.y - s1 * p4.y), sign);
           X(0) = cadd(x0, cadd(p1, p2));
           X(1) = cadd(a1, b1);
           X(2) = cadd(a2, b2);
@@ -169,8 +188,12 @@ constexpr int TM = 4;   // complex lines (y wavenumbers) per z-FFT block
 // tile: (TK consecutive interleaved positions, z index i); wave w transforms
 // lines 2w and 2w+1 of the tile.  Thread t loads slot q = t % TK of the rows
 // j = t / TK + 32 it, so its k, neighbour slot and udx3m stay fixed.
+template <typename R>
 __global__ void __launch_bounds__(256) k_yfft_forward(PoissonArgs a) {
-  extern __shared__ double2 lds[];
+  using C = cplx_t<R>;
+  extern __shared__ double2 lds_raw[];
+  C *lds = reinterpret_cast<C *>(lds_raw);
+  C *S = reinterpret_cast<C *>(a.S);
   const Geom g = a.g;
   const RealPlan &py = a.py;
   const int H = py.H, N = py.N;
@@ -218,25 +241,25 @@ __global__ void __launch_bounds__(256) k_yfft_forward(PoissonArgs a) {
         double x = z0[b];
         if (a.from_velocity)
           x = ((z1[b] - z0[b]) * sc.dz + (y1[b] - y0[b]) * sc.dy + (x1[b] - x0[b]) * u3) * usdtal;
-        const double v = live ? x : 0.0;
-        double2 *e = &lds[q * H + swz(rv[b], hsw)];
-        if (!py.half) *e = make_double2(v, 0.0);
+        const R v = live ? (R)x : (R)0;   // divergence in FP64, spectrum in R
+        C *e = &lds[q * H + swz(rv[b], hsw)];
+        if (!py.half) *e = mkc(v, (R)0);
         else if (j & 1) e->y = v;
         else e->x = v;
       }
     }
   }
   __syncthreads();
-  lds_fft_wave(lds + (threadIdx.x / 64) * (TK / 4) * H, TK / 4, py.c, -1.0, threadIdx.x % 64);
+  lds_fft_wave<R>(lds + (threadIdx.x / 64) * (TK / 4) * H, TK / 4, py.c, (R)-1, threadIdx.x % 64);
   __syncthreads();
   const int nymh = a.nymh;
   for (int base = threadIdx.x; base < TK * nymh; base += 256 * LB) {
-    double2 pw[LB];
+    C pw[LB];
 #pragma unroll
     for (int b = 0; b < LB; b++) {
       const int idx = min(base + 256 * b, TK * nymh - 1);
       const int m = idx % nymh;
-      pw[b] = py.half ? py.post[m] : make_double2(1.0, 0.0);
+      pw[b] = py.half ? cvt<R>(py.post[m]) : mkc((R)1, (R)0);
     }
 #pragma unroll
     for (int b = 0; b < LB; b++) {
@@ -245,36 +268,39 @@ __global__ void __launch_bounds__(256) k_yfft_forward(PoissonArgs a) {
       const int q = idx / nymh, m = idx - q * nymh;
       const int k = k_of_pos(s0 + q, g.L);
       if (k >= g.nxm) continue;
-      double2 X;
+      C X;
       if (py.half) {
-        double2 zm = lds[q * H + swz(m == H ? 0 : m, hsw)];
-        double2 zc = conj2(lds[q * H + swz(m == 0 ? 0 : H - m, hsw)]);
-        double2 E = make_double2(0.5 * (zm.x + zc.x), 0.5 * (zm.y + zc.y));
-        double2 D = csub(zm, zc);
-        double2 O = make_double2(0.5 * D.y, -0.5 * D.x);  // D / (2i)
+        C zm = lds[q * H + swz(m == H ? 0 : m, hsw)];
+        C zc = conj2(lds[q * H + swz(m == 0 ? 0 : H - m, hsw)]);
+        C E = mkc((R)0.5 * (zm.x + zc.x), (R)0.5 * (zm.y + zc.y));
+        C D = csub(zm, zc);
+        C O = mkc((R)0.5 * D.y, (R)-0.5 * D.x);  // D / (2i)
         X = cadd(E, cmul(pw[b], O));
       } else {
         X = lds[q * H + swz(m, hsw)];
       }
-      a.S[((int64_t)k * g.nzm + i) * a.ldm + m] = X;
+      S[((int64_t)k * g.nzm + i) * a.ldm + m] = X;
     }
   }
 }
 
 // ---------------------------------------------------------------- z transforms
 // tile: (TM wavenumbers m, k); wave w transforms line w
-__global__ void __launch_bounds__(256) k_zfft(PoissonArgs a, double sign) {
-  extern __shared__ double2 lds[];
+template <typename R>
+__global__ void __launch_bounds__(256) k_zfft(PoissonArgs a, R sign) {
+  using C = cplx_t<R>;
+  extern __shared__ double2 lds_raw[];
+  C *lds = reinterpret_cast<C *>(lds_raw);
   const int N = a.pz.n;
   const int ntile = (a.nymh + TM - 1) / TM;
   const int64_t tile = fft_tile((int64_t)ntile * a.g.nxm);
   if (tile < 0 || (a.ds && a.ds->halted)) return;
   const int m0 = (int)(tile % ntile) * TM, k = (int)(tile / ntile);
   const bool zsw = (N & 7) == 0;
-  double2 *S = a.S + (int64_t)k * a.g.nzm * a.ldm;
+  C *S = reinterpret_cast<C *>(a.S) + (int64_t)k * a.g.nzm * a.ldm;
   const int tot = TM * N;
   for (int base = threadIdx.x; base < tot; base += 256 * LB) {
-    double2 v[LB];
+    C v[LB];
     int rv[LB];
 #pragma unroll
     for (int b = 0; b < LB; b++) {   // all loads of the batch first
@@ -288,11 +314,11 @@ __global__ void __launch_bounds__(256) k_zfft(PoissonArgs a, double sign) {
       const int idx = base + 256 * b;
       if (idx >= tot) continue;
       const int q = idx % TM;
-      lds[q * N + swz(rv[b], zsw)] = m0 + q < a.nymh ? v[b] : make_double2(0.0, 0.0);
+      lds[q * N + swz(rv[b], zsw)] = m0 + q < a.nymh ? v[b] : mkc((R)0, (R)0);
     }
   }
   __syncthreads();
-  lds_fft_wave(lds + (threadIdx.x / 64) * N, 1, a.pz, sign, threadIdx.x % 64);
+  lds_fft_wave<R>(lds + (threadIdx.x / 64) * N, 1, a.pz, sign, threadIdx.x % 64);
   __syncthreads();
   for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
     const int q = idx % TM, nn = idx / TM;
@@ -310,13 +336,15 @@ __global__ void __launch_bounds__(256) k_zfft(PoissonArgs a, double sign) {
 // The recurrences walk k in blocks of XB rows whose spectrum loads are issued
 // together before the block's rows are eliminated.
 constexpr int XB = 8;
+template <typename R>
 __global__ void __launch_bounds__(256) k_xsolve_pressure(PoissonArgs a) {
+  using C = cplx_t<R>;
   const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int nymh = a.nymh, nzm = a.g.nzm, nxm = a.g.nxm;
   if (id >= (int64_t)nymh * nzm || (a.ds && a.ds->halted)) return;
   const int n = (int)(id / nymh), m = (int)(id - (int64_t)n * nymh);
   const int64_t stride = (int64_t)nzm * a.ldm;
-  double2 *S = a.S + (int64_t)n * a.ldm + m;
+  C *S = reinterpret_cast<C *>(a.S) + (int64_t)n * a.ldm + m;
   double *cp = a.cp + (int64_t)n * a.ldm + m;
   const double lam = a.ak2[m] + 0.0;  // ak2 indexed by y wavenumber
   const double ak1n = a.ak1[n];
@@ -327,7 +355,7 @@ __global__ void __launch_bounds__(256) k_xsolve_pressure(PoissonArgs a) {
   for (int k0 = 0; k0 < nxm; k0 += XB) {
     double2 rb[XB];
 #pragma unroll
-    for (int t = 0; t < XB; t++) rb[t] = S[min(k0 + t, nxm - 1) * stride];
+    for (int t = 0; t < XB; t++) rb[t] = widen(S[min(k0 + t, nxm - 1) * stride]);
 #pragma unroll
     for (int t = 0; t < XB; t++) {
       const int k = k0 + t;
@@ -348,18 +376,18 @@ __global__ void __launch_bounds__(256) k_xsolve_pressure(PoissonArgs a) {
       u_prev = u;
       du_prev = du;
       double ui = (zero_mode && k == nxm - 1) ? 0.0 : 1.0 / u;
-      S[k * stride] = make_double2(r.x * ui, r.y * ui);
+      S[k * stride] = narrow<C>(make_double2(r.x * ui, r.y * ui));
       cp[k * stride] = du * ui;
     }
   }
-  double2 x = S[(nxm - 1) * stride];
+  double2 x = widen(S[(nxm - 1) * stride]);
   for (int k1 = nxm - 2; k1 >= 0; k1 -= XB) {
     double2 yb[XB];
     double cb[XB];
 #pragma unroll
     for (int t = 0; t < XB; t++) {
       const int k = max(k1 - t, 0);
-      yb[t] = S[k * stride];
+      yb[t] = widen(S[k * stride]);
       cb[t] = cp[k * stride];
     }
 #pragma unroll
@@ -367,14 +395,17 @@ __global__ void __launch_bounds__(256) k_xsolve_pressure(PoissonArgs a) {
       const int k = k1 - t;
       if (k < 0) break;
       x = make_double2(yb[t].x - cb[t] * x.x, yb[t].y - cb[t] * x.y);
-      S[k * stride] = x;
+      S[k * stride] = narrow<C>(x);
     }
   }
 }
 
 // ---------------------------------------------------------------- y inverse
+template <typename R>
 __global__ void __launch_bounds__(256) k_yfft_inverse(PoissonArgs a) {
-  extern __shared__ double2 lds[];
+  using C = cplx_t<R>;
+  extern __shared__ double2 lds_raw[];
+  C *lds = reinterpret_cast<C *>(lds_raw);
   const Geom g = a.g;
   const RealPlan &py = a.py;
   const int H = py.H, N = py.N;
@@ -386,22 +417,22 @@ __global__ void __launch_bounds__(256) k_yfft_inverse(PoissonArgs a) {
   const bool hsw = (H & 7) == 0;
   const int nymh = a.nymh;
   for (int base = threadIdx.x; base < TK * H; base += 256 * LB) {
-    double2 xa[LB], xb[LB], pw[LB];
+    C xa[LB], xb[LB], pw[LB];
     int rv[LB];
 #pragma unroll
     for (int b = 0; b < LB; b++) {   // all loads of the batch first
       const int ic = min(base + 256 * b, TK * H - 1);
       const int q = ic / H, t = ic - q * H;
       const int k = min(k_of_pos(s0 + q, g.L), g.nxm - 1);
-      const double2 *X = a.S + ((int64_t)k * g.nzm + i) * a.ldm;
+      const C *X = reinterpret_cast<const C *>(a.S) + ((int64_t)k * g.nzm + i) * a.ldm;
       if (py.half) {
         xa[b] = X[t];
         xb[b] = X[H - t];
-        pw[b] = py.post[t];
+        pw[b] = cvt<R>(py.post[t]);
       } else {
         xa[b] = X[min(t, nymh - 1)];
         xb[b] = X[min(N - t, nymh - 1)];
-        pw[b] = make_double2(1.0, 0.0);
+        pw[b] = mkc((R)1, (R)0);
       }
       rv[b] = py.c.rev[t];
     }
@@ -411,27 +442,27 @@ __global__ void __launch_bounds__(256) k_yfft_inverse(PoissonArgs a) {
       if (idx >= TK * H) continue;
       const int q = idx / H, t = idx - q * H;
       const bool live = k_of_pos(s0 + q, g.L) < g.nxm;
-      double2 z;
+      C z;
       if (py.half) {
-        double2 xm = xa[b], xh = xb[b];
-        if (t == 0) { xm.y = 0.0; xh.y = 0.0; }  // DC and Nyquist: imaginary parts ignored
-        double2 xc = conj2(xh);
-        double2 E = cadd(xm, xc);
-        double2 O = cmul(csub(xm, xc), conj2(pw[b]));
-        z = make_double2(E.x - O.y, E.y + O.x);  // E + i O
+        C xm = xa[b], xh = xb[b];
+        if (t == 0) { xm.y = (R)0; xh.y = (R)0; }  // DC and Nyquist: imaginary parts ignored
+        C xc = conj2(xh);
+        C E = cadd(xm, xc);
+        C O = cmul(csub(xm, xc), conj2(pw[b]));
+        z = mkc(E.x - O.y, E.y + O.x);  // E + i O
       } else {
         if (t < nymh) {
           z = xa[b];
-          if (t == 0) z.y = 0.0;
+          if (t == 0) z.y = (R)0;
         } else {
           z = conj2(xb[b]);
         }
       }
-      lds[q * H + swz(rv[b], hsw)] = live ? z : make_double2(0.0, 0.0);
+      lds[q * H + swz(rv[b], hsw)] = live ? z : mkc((R)0, (R)0);
     }
   }
   __syncthreads();
-  lds_fft_wave(lds + (threadIdx.x / 64) * (TK / 4) * H, TK / 4, py.c, +1.0, threadIdx.x % 64);
+  lds_fft_wave<R>(lds + (threadIdx.x / 64) * (TK / 4) * H, TK / 4, py.c, (R)1, threadIdx.x % 64);
   __syncthreads();
   for (int idx = threadIdx.x; idx < TK * N; idx += blockDim.x) {
     const int q = idx % TK, j = idx / TK;
@@ -439,7 +470,7 @@ __global__ void __launch_bounds__(256) k_yfft_inverse(PoissonArgs a) {
     if (k >= g.nxm) continue;
     double v;
     if (py.half) {
-      double2 z = lds[q * H + swz(j >> 1, hsw)];
+      C z = lds[q * H + swz(j >> 1, hsw)];
       v = (j & 1) ? z.y : z.x;
     } else {
       v = lds[q * H + swz(j, hsw)].x;
@@ -452,37 +483,48 @@ __global__ void __launch_bounds__(256) k_yfft_inverse(PoissonArgs a) {
 int fft_max_len() { return 4096; }
 
 // Dynamic LDS above 64 KiB must be opted into per kernel (gfx950: 160 KiB per workgroup).
+template <typename R>
+static hipError_t fft_configure_lds_r(size_t bytes) {
+  hipError_t e = hipFuncSetAttribute((const void *)k_yfft_forward<R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute((const void *)k_yfft_inverse<R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute((const void *)k_zfft<R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
 hipError_t fft_configure_lds(size_t bytes) {
-  hipError_t e = hipFuncSetAttribute((const void *)k_yfft_forward, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-  if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute((const void *)k_yfft_inverse, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-  if (e != hipSuccess) return e;
-  return hipFuncSetAttribute((const void *)k_zfft, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  hipError_t e = fft_configure_lds_r<double>(bytes);
+  return e != hipSuccess ? e : fft_configure_lds_r<float>(bytes);
 }
 
 static dim3 tile_grid(int64_t total) { return dim3((unsigned)(((total + 7) / 8) * 8)); }
+static size_t cplx_bytes(const PoissonArgs &a) { return a.spec32 ? sizeof(float2) : sizeof(double2); }
 
 hipError_t launch_yfft_forward(const PoissonArgs &a, hipStream_t st) {
   dim3 grid = tile_grid((int64_t)(a.g.ldx / TK) * a.g.nzm);
-  size_t shm = sizeof(double2) * (size_t)TK * a.py.H;
-  hipLaunchKernelGGL(k_yfft_forward, grid, dim3(256), shm, st, a);
+  size_t shm = cplx_bytes(a) * (size_t)TK * a.py.H;
+  if (a.spec32) hipLaunchKernelGGL(k_yfft_forward<float>, grid, dim3(256), shm, st, a);
+  else hipLaunchKernelGGL(k_yfft_forward<double>, grid, dim3(256), shm, st, a);
   return hipGetLastError();
 }
 hipError_t launch_zfft(const PoissonArgs &a, int sign, hipStream_t st) {
   dim3 grid = tile_grid((int64_t)((a.nymh + TM - 1) / TM) * a.g.nxm);
-  size_t shm = sizeof(double2) * (size_t)TM * a.pz.n;
-  hipLaunchKernelGGL(k_zfft, grid, dim3(256), shm, st, a, (double)sign);
+  size_t shm = cplx_bytes(a) * (size_t)TM * a.pz.n;
+  if (a.spec32) hipLaunchKernelGGL(k_zfft<float>, grid, dim3(256), shm, st, a, (float)sign);
+  else hipLaunchKernelGGL(k_zfft<double>, grid, dim3(256), shm, st, a, (double)sign);
   return hipGetLastError();
 }
 hipError_t launch_xsolve_pressure(const PoissonArgs &a, hipStream_t st) {
   int64_t n = (int64_t)a.nymh * a.g.nzm;
-  hipLaunchKernelGGL(k_xsolve_pressure, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a);
+  dim3 grid((unsigned)((n + 255) / 256));
+  if (a.spec32) hipLaunchKernelGGL(k_xsolve_pressure<float>, grid, dim3(256), 0, st, a);
+  else hipLaunchKernelGGL(k_xsolve_pressure<double>, grid, dim3(256), 0, st, a);
   return hipGetLastError();
 }
 hipError_t launch_yfft_inverse(const PoissonArgs &a, hipStream_t st) {
   dim3 grid = tile_grid((int64_t)(a.g.ldx / TK) * a.g.nzm);
-  size_t shm = sizeof(double2) * (size_t)TK * a.py.H;
-  hipLaunchKernelGGL(k_yfft_inverse, grid, dim3(256), shm, st, a);
+  size_t shm = cplx_bytes(a) * (size_t)TK * a.py.H;
+  if (a.spec32) hipLaunchKernelGGL(k_yfft_inverse<float>, grid, dim3(256), shm, st, a);
+  else hipLaunchKernelGGL(k_yfft_inverse<double>, grid, dim3(256), shm, st, a);
   return hipGetLastError();
 }
 

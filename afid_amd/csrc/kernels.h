@@ -16,6 +16,7 @@ struct FusedArgs {
   StepScalars sc;
   const DevStep *ds;                 // non-null: scalars of substep ns come from the device
   int ns;
+  int h32;                           // histories stored in FP32 (mixed precision)
 };
 
 struct RoutineArgs {
@@ -85,6 +86,7 @@ struct PoissonArgs {
   int from_velocity;     // 1: y-FFT input is div(u)/(dt al) computed on the fly; 0: dph
   const DevStep *ds;
   int ns;
+  int spec32;            // S holds FP32 complex values and the y/z FFTs run in FP32 (mixed precision)
 };
 
 hipError_t launch_fused_update(const FusedArgs &a, hipStream_t st);
@@ -140,6 +142,8 @@ struct InterpArgs {
   const double *crd;    // 3 per entry
 };
 hipError_t launch_interp(const InterpArgs &a, hipStream_t st);
+// n FP64 values to FP32 (to_f32) or back, element for element.
+hipError_t launch_convert(const void *src, void *dst, int64_t n, int to_f32, hipStream_t st);
 hipError_t launch_slabs(const FieldPtrs &f, const Geom &g, const int *kslab, int nslab, double *out,
                         hipStream_t st);
 

@@ -324,6 +324,17 @@ hipError_t launch_interp(const InterpArgs &a, hipStream_t st) {
   return hipGetLastError();
 }
 
+__global__ void k_convert(const void *src, void *dst, int64_t n, int to_f32) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    if (to_f32) static_cast<float *>(dst)[e] = (float)static_cast<const double *>(src)[e];
+    else static_cast<double *>(dst)[e] = (double)static_cast<const float *>(src)[e];
+  }
+}
+hipError_t launch_convert(const void *src, void *dst, int64_t n, int to_f32, hipStream_t st) {
+  hipLaunchKernelGGL(k_convert, dim3(4096), dim3(256), 0, st, src, dst, n, to_f32);
+  return hipGetLastError();
+}
+
 static hipError_t finalize(const ReduceArgs &a, int nq, int maxmask, int minmask, hipStream_t st) {
   hipLaunchKernelGGL(k_finalize, dim3(nq), dim3(256), 0, st, a.partial, a.out, nq, (const int *)nullptr,
                      maxmask, minmask);

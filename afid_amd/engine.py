@@ -46,7 +46,10 @@ class _Params(ctypes.Structure):
                 ("nsst", ctypes.c_int), ("istr3", ctypes.c_int), ("str3", ctypes.c_double),
                 ("alx3", ctypes.c_double), ("ylen", ctypes.c_double), ("zlen", ctypes.c_double),
                 ("ray", ctypes.c_double), ("pra", ctypes.c_double), ("inslws", ctypes.c_int),
-                ("inslwn", ctypes.c_int), ("tempbp", ctypes.c_double), ("temptp", ctypes.c_double)]
+                ("inslwn", ctypes.c_int), ("tempbp", ctypes.c_double), ("temptp", ctypes.c_double),
+                ("precision", ctypes.c_int)]
+
+PREC_FP64, PREC_MIXED = 0, 1
 
 
 _lib = None
@@ -204,11 +207,12 @@ class CellParams:
     inslwn: int = 1
     tempbp: float = 1.0
     temptp: float = 0.0
+    precision: int = PREC_FP64     # PREC_MIXED: FP32 histories and FFT spectrum
 
     def to_c(self) -> _Params:
         return _Params(self.nxm, self.nym, self.nzm, self.nsst, self.istr3, self.str3, self.alx3,
                        self.ylen, self.zlen, self.ray, self.pra, self.inslws, self.inslwn,
-                       self.tempbp, self.temptp)
+                       self.tempbp, self.temptp, self.precision)
 
     @property
     def cells(self) -> int:

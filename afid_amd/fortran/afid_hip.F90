@@ -21,6 +21,7 @@
       integer(c_int), parameter :: AFID_ERR_IO = -6, AFID_ERR_GRID = -7
       integer(c_int), parameter :: AFID_NUM_PIVOT = 1
       integer(c_int), parameter :: AFID_NUM_SINGULAR = 2
+      integer(c_int), parameter :: AFID_PREC_FP64 = 0, AFID_PREC_MIXED = 1
 
 !     field identifiers (local_arrays, param.F90:79-87)
       integer(c_int), parameter :: AFID_VX = 0, AFID_VY = 1
@@ -37,6 +38,7 @@
         real(c_double) :: str3, alx3, ylen, zlen, ray, pra
         integer(c_int) :: inslws, inslwn
         real(c_double) :: tempbp, temptp
+        integer(c_int) :: precision
       end type afid_params_t
 
       interface

@@ -292,6 +292,7 @@
       p%inslwn = inslwn
       p%tempbp = 1.d0
       p%temptp = 0.d0
+      p%precision = AFID_PREC_FP64
       call afid_check(afid_create(ctx, p, c_null_ptr), 'afid_create')
       if (nread /= 0) then
         write(6,*) 'Reading initial condition from file'

@@ -138,6 +138,55 @@ def bench_sweep(args):
     print(json.dumps(out))
 
 
+def bench_mixed(args):
+    """Mixed-precision cell against the FP64 run of the same seeded input:
+    nxm=256 (istr3=6, str3=6), ny=nz=512, aspect 2, Ra=1e7, Pr=0.7, CFL dt;
+    throughput of both and the per-field rel-L2 difference after the steps."""
+    import numpy as np
+    from afid_amd import Cell, CellParams
+    from afid_amd.engine import PREC_FP64, PREC_MIXED
+    p = dict(nxm=256, nym=512, nzm=512, istr3=6, str3=6.0, ylen=2.0, zlen=2.0, ray=1e7, pra=0.7)
+    cells = p["nxm"] * p["nym"] * p["nzm"]
+    ctl = (1, 1e-3, 1e-8, 1e-2, 1.2)
+    res, fields = {}, {}
+    for name, prec in (("fp64", PREC_FP64), ("mixed", PREC_MIXED)):
+        print(f"mixed: {name} run, {args.mixed_steps} steps", file=sys.stderr, flush=True)
+        c = Cell(CellParams(nsst=3, alx3=1.0, precision=prec, **p))
+        c.init_linear_noise(3, 1e-3)
+        c.steps(0, 2, *ctl)             # captures the step graph; dt = dtmin for these two
+        c.synchronize()
+        t0 = time.perf_counter()
+        err, _, done = c.steps(2, args.mixed_steps - 2, *ctl)
+        c.synchronize()
+        el = time.perf_counter() - t0
+        if err or done != args.mixed_steps - 2:
+            raise RuntimeError(f"{name}: step control stopped: errorcode {err} after {done} steps")
+        fields[name] = {nm: c.download(nm) for nm in ("vx", "vy", "vz", "temp", "pr")}
+        res[name] = {"Mcell_updates_per_s": round(cells * done / el / 1e6, 2), "ms_per_step": round(el / done * 1e3, 3),
+                     "max_divergence": c.CheckDivergence(), "time": c.time}
+        c.close()
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    nxm = p["nxm"]
+    err = {nm: rel(fields["mixed"][nm], fields["fp64"][nm]) for nm in ("vx", "vy", "vz", "temp")}
+    pm, pf = fields["mixed"]["pr"][:, :, :nxm], fields["fp64"]["pr"][:, :, :nxm]
+    err["pr"] = rel(pm - pm.mean(), pf - pf.mean())
+    out = {
+        "metric": "Mcell-updates/s", "value": res["mixed"]["Mcell_updates_per_s"], "unit": "Mcell-updates/s",
+        "n_gpus": 1, "steps": args.mixed_steps, "warmup": 2, "ms_per_step": res["mixed"]["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32",
+        "data": "synthetic (seeded conduction + noise)",
+        "config": {"workload": f"mixed precision: RB cell {p['nxm']}x{p['nym']}x{p['nzm']} istr3=6 str3=6 aspect 2 "
+                               f"Ra=1e7 Pr=0.7 RK3 CFL dt, {args.mixed_steps} steps; FP32 histories and FFT "
+                               "spectrum, one refinement sweep per step", "cells": cells},
+        "fp64": res["fp64"], "mixed": res["mixed"],
+        "speedup_vs_fp64": round(res["mixed"]["Mcell_updates_per_s"] / res["fp64"]["Mcell_updates_per_s"], 3),
+        "rel_l2_vs_fp64": err,
+    }
+    print(json.dumps(out))
+
+
 def bench_serve(args):
     """Serving: Poisson arrivals (4/s) admitted by a 200 GB device budget."""
     from afid_amd.serve import Server, serving_requests, summarize
@@ -174,9 +223,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
     ap.add_argument("--profile-steps", type=int, default=2)
-    ap.add_argument("--workload", default="cell", choices=("cell", "sweep", "serve"),
+    ap.add_argument("--workload", default="cell", choices=("cell", "sweep", "serve", "mixed"),
                     help="cell: one production cell (default); sweep: 64-cell batched parameter sweep; "
-                         "serve: Poisson request stream")
+                         "serve: Poisson request stream; mixed: mixed-precision cell against FP64")
+    ap.add_argument("--mixed-steps", type=int, default=200)
     ap.add_argument("--sweep-cells", type=int, default=64)
     ap.add_argument("--sweep-steps", type=int, default=500)
     ap.add_argument("--requests", type=int, default=24)
@@ -189,6 +239,8 @@ def main():
         return bench_sweep(args)
     if args.workload == "serve":
         return bench_serve(args)
+    if args.workload == "mixed":
+        return bench_mixed(args)
 
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)

@@ -50,7 +50,20 @@ typedef struct afid_params {
   double ray, pra;     /* Rayleigh, Prandtl                                    */
   int inslws, inslwn;  /* 1 no-slip, 0 stress-free: lower / upper plate        */
   double tempbp, temptp; /* plate temperatures, SetTempBCs.F90:15-20 (1, 0)   */
+  int precision;       /* AFID_PREC_FP64 (the reference's) or AFID_PREC_MIXED  */
 } afid_params;
+
+/* Precision modes.  MIXED (no counterpart in the reference, which is FP64
+ * throughout): the explicit-term histories rux..rutemp are stored in FP32 and
+ * the y/z FFTs of the pressure solve run on an FP32 spectrum; the fields, the
+ * wall-normal factorisations and solves, the pressure x-solve arithmetic and
+ * all reductions stay FP64, and each step ends with one refinement sweep: the
+ * divergence left after the last correction, computed in FP64, is solved for
+ * and corrected once more.  Needs the default split update (not
+ * AFID_UPDATE=fused); the per-routine ImplicitAndUpdate entry points return
+ * AFID_ERR_STATE in this mode. */
+#define AFID_PREC_FP64 0
+#define AFID_PREC_MIXED 1
 
 /* Optional caller-computed 1-D metrics, Fortran 1-based arrays as declared in
  * param.F90:24-48 (pass the address of element 1).  Any NULL member is
