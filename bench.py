@@ -164,6 +164,14 @@ def bench_mixed(args):
         fields[name] = {nm: c.download(nm) for nm in ("vx", "vy", "vz", "temp", "pr")}
         res[name] = {"Mcell_updates_per_s": round(cells * done / el / 1e6, 2), "ms_per_step": round(el / done * 1e3, 3),
                      "max_divergence": c.CheckDivergence(), "time": c.time}
+        # per-kernel HIP-event times over a few host-driven steps after the comparison
+        c.profile(True)
+        c.profile_reset()
+        for n in range(args.profile_steps):
+            c.step(args.mixed_steps + n, *ctl)
+        prof = c.profile_read()
+        c.profile(False)
+        res[name]["kernel_ms_per_step"] = {k: round(v[0] / args.profile_steps, 3) for k, v in prof.items() if v[1]}
         c.close()
 
     def rel(a, b):
