@@ -562,7 +562,7 @@ __global__ void __launch_bounds__(WG)
 // The vz increment at k=1, q_new - q_old, becomes the new stale row and the
 // temperature's rhs(1).
 constexpr int STENCIL_WG = 256;
-constexpr int ZM = 16;   // z planes per block
+constexpr int ZM = 32;   // z planes per block
 // Coefficient rows staged in LDS by k_stencil_rhs: eight per-k metric rows,
 // then scale, dl, du of the x, y/z and temperature systems.
 enum StencilCoef {
