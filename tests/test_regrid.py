@@ -79,7 +79,9 @@ def test_gridnew_faces():
     assert rm[-1] == 2 * rc[-1] - rm[-2]
     for nxm, s in ((24, 6.0), (31, 3.0)):
         rc, rm = O.gridnew(nxm + 1, 1.0, 6, s)
-        xc = Oracle(nxm=nxm, nym=8, nzm=8, istr3=6, str3=s, alx3=1.0).arr("xc")[1:nxm + 2]
+        o = Oracle(nxm=nxm, nym=8, nzm=8, istr3=6, str3=s, alx3=1.0)
+        xc = o.arr("xc")[1:nxm + 2].copy()      # a view into the oracle's memory: copy before closing
+        o.close()
         np.testing.assert_allclose(rc, xc, rtol=1e-14, atol=1e-16)
         assert np.all(np.diff(rc) > 0)
 
