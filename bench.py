@@ -51,14 +51,16 @@ SUBSTEP_WORDS = 65
 # kernel name: the PMC kernels it stands for and their weights per substep.
 PMC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_cfg2_384x768x768.json")
 PMC_KERNELS = {
-    "stencil_rhs": [("k_stencil_rhs<u32, F>", 1 / 3), ("k_stencil_rhs<u32, T>", 2 / 3)],
+    # k_stencil_rhs<offset type, histories read, device scalars, FP32 histories>: the timed
+    # graph steps run the device-scalar instances
+    "stencil_rhs": [("k_stencil_rhs<u32, F, T, F>", 1 / 3), ("k_stencil_rhs<u32, T, T, F>", 2 / 3)],
     "column_solve": [("k_column_solve<25, u32>", 1.0)],
     "correct_vel_pr": [("k_correct", 1.0)],
-    "div_yfft_r2c": [("k_yfft_forward", 1.0)],
-    "zfft_fwd": [("k_zfft", 1.0)],
-    "zfft_bwd": [("k_zfft", 1.0)],
-    "xsolve_pressure": [("k_xsolve_pressure", 1.0)],
-    "yfft_c2r": [("k_yfft_inverse", 1.0)],
+    "div_yfft_r2c": [("k_yfft_forward<double>", 1.0)],
+    "zfft_fwd": [("k_zfft<double>", 1.0)],
+    "zfft_bwd": [("k_zfft<double>", 1.0)],
+    "xsolve_pressure": [("k_xsolve_pressure<double>", 1.0)],
+    "yfft_c2r": [("k_yfft_inverse<double>", 1.0)],
 }
 
 
