@@ -257,6 +257,94 @@
           import :: c_int, c_ptr
           type(c_ptr), value :: ctx
         end function
+!       whole main.F90 steps on the device, one captured graph per step
+        integer(c_int) function afid_steps(ctx, ntime0, nsteps, idtv,   &
+     &      dt, dtmin, dtmax, limitcfl, dt_last, errorcode, steps_done)  &
+     &      bind(C, name='afid_steps')
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: ctx
+          integer(c_int), value :: ntime0, nsteps, idtv
+          real(c_double), value :: dt, dtmin, dtmax, limitcfl
+          real(c_double), intent(out) :: dt_last
+          integer(c_int), intent(out) :: errorcode, steps_done
+        end function
+        integer(c_int) function afid_last_hip_error(ctx)                &
+     &      bind(C, name='afid_last_hip_error')
+          import :: c_int, c_ptr
+          type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function afid_get_stale(ctx, plane)              &
+     &      bind(C, name='afid_get_stale')
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: ctx
+          real(c_double), intent(out) :: plane(*)
+        end function
+        integer(c_int) function afid_set_stale(ctx, plane)              &
+     &      bind(C, name='afid_set_stale')
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: ctx
+          real(c_double), intent(in) :: plane(*)
+        end function
+        integer(c_int) function afid_get_stats(ctx, out, nsamples)      &
+     &      bind(C, name='afid_get_stats')
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: ctx
+          real(c_double), intent(out) :: out(*)
+          integer(c_int), intent(out) :: nsamples
+        end function
+        integer(c_int) function afid_reset_stats(ctx)                   &
+     &      bind(C, name='afid_reset_stats')
+          import :: c_int, c_ptr
+          type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function afid_get_slabs(ctx, kslab, nslab, out)  &
+     &      bind(C, name='afid_get_slabs')
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: ctx
+          integer(c_int), intent(in) :: kslab(*)
+          integer(c_int), value :: nslab
+          real(c_double), intent(out) :: out(*)
+        end function
+!       concurrent cells: asynchronous steps and interleaved batches
+        integer(c_int) function afid_steps_async(ctx, ntime0, nsteps,    &
+     &      idtv, dt, dtmin, dtmax, limitcfl)                            &
+     &      bind(C, name='afid_steps_async')
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: ctx
+          integer(c_int), value :: ntime0, nsteps, idtv
+          real(c_double), value :: dt, dtmin, dtmax, limitcfl
+        end function
+        integer(c_int) function afid_steps_more(ctx, nsteps)           &
+     &      bind(C, name='afid_steps_more')
+          import :: c_int, c_ptr
+          type(c_ptr), value :: ctx
+          integer(c_int), value :: nsteps
+        end function
+        integer(c_int) function afid_steps_query(ctx, busy)            &
+     &      bind(C, name='afid_steps_query')
+          import :: c_int, c_ptr
+          type(c_ptr), value :: ctx
+          integer(c_int), intent(out) :: busy
+        end function
+        integer(c_int) function afid_steps_wait(ctx, dt_last,           &
+     &      errorcode, steps_done) bind(C, name='afid_steps_wait')
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: ctx
+          real(c_double), intent(out) :: dt_last
+          integer(c_int), intent(out) :: errorcode, steps_done
+        end function
+        integer(c_int) function afid_batch_steps(cells, ncells, nsteps) &
+     &      bind(C, name='afid_batch_steps')
+          import :: c_int, c_ptr
+          type(c_ptr), intent(in) :: cells(*)
+          integer(c_int), value :: ncells, nsteps
+        end function
+        integer(c_int) function afid_estimate_bytes(p, nbytes)          &
+     &      bind(C, name='afid_estimate_bytes')
+          import :: c_int, c_int64_t, afid_params_t
+          type(afid_params_t), intent(in) :: p
+          integer(c_int64_t), intent(out) :: nbytes
+        end function
 !       restart regridding (interp.F90)
         integer(c_int) function afid_interp_field(ctx, field, old, nxo,  &
      &      nyo, nzo, istro3, stro3) bind(C, name='afid_interp_field')

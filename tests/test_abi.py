@@ -54,3 +54,17 @@ def test_no_oracle_in_product():
             if f.endswith((".py", ".hip", ".h", ".cpp", ".F90")):
                 src = open(os.path.join(dirpath, f), errors="ignore").read()
                 assert not re.search(r"\boracle\b|afo_|liboracle", src), f
+
+
+def test_fortran_binding_covers_the_abi():
+    """Every entry point of include/afid.h and include/afid_io.h has a bind(C)
+    interface in the Fortran module, except the profiling and test hooks."""
+    import os
+    import re
+    src = open(os.path.join(os.path.dirname(engine.LIB_PATH), "..", "fortran", "afid_hip.F90")).read()
+    bound = set(re.findall(r"name='(afid_\w+)'", src))
+    hooks = {"afid_profile_enable", "afid_profile_read", "afid_profile_reset", "afid_pressure_spectrum",
+             "afid_h5_write_field", "afid_h5_read_field"}
+    declared = set(engine.exported_symbols()) | set(engine.exported_symbols("afid_io.h"))
+    assert sorted(declared - bound - hooks) == []
+    assert bound <= declared
