@@ -12,10 +12,12 @@
 !  normal exit); StatRoutines.F90:12-44 and SlabDumpRoutines.F90:12-70
 !  at the output times after tsta (main.F90:210-216), slabs into stst3/.
 !
-!  Usage: afid_run [bou.in] [--seed N] [--routines] [--out DIR]
+!  Usage: afid_run [bou.in] [--seed N] [--routines] [--mixed] [--out DIR]
 !                  [--parse-only] [--abi-check]
 !    --routines   TimeMarcher calls the per-routine entry points in the
 !                 reference's order instead of the fused substep
+!    --mixed      mixed-precision mode (FP32 histories and FFT spectrum,
+!                 one refinement sweep per step); not with --routines
 !    --out DIR    final fields vx, vy, vz, temp, pr are written there as
 !                 raw FP64 stream files, Fortran order (nx, nym, nzm)
 !    --parse-only print the parsed bou.in and stop (no device use)
@@ -38,6 +40,7 @@
       type(c_ptr) :: ctx = c_null_ptr
       real(8) :: time = 0.d0
       logical :: per_routine = .false.
+      logical :: mixed = .false.
       integer(c_int64_t) :: seed = 1
 
       contains
@@ -237,6 +240,8 @@
           read(arg,*) seed
         case ('--routines')
           per_routine = .true.
+        case ('--mixed')
+          mixed = .true.
         case ('--out')
           ia = ia + 1
           call get_command_argument(ia, outdir)
@@ -293,6 +298,7 @@
       p%tempbp = 1.d0
       p%temptp = 0.d0
       p%precision = AFID_PREC_FP64
+      if (mixed) p%precision = AFID_PREC_MIXED
       call afid_check(afid_create(ctx, p, c_null_ptr), 'afid_create')
       if (nread /= 0) then
         write(6,*) 'Reading initial condition from file'

@@ -129,3 +129,26 @@ def test_fortran_stats_and_slabs(tmp_path):
     assert h5_values(tmp_path / "stafield_master.h5", "averaging_time")[0] == nout
     assert len(os.listdir(tmp_path / "stst3")) % 36 == 0 and len(os.listdir(tmp_path / "stst3")) >= 36
     np.testing.assert_array_equal(h5_values(tmp_path / "stafield_master.h5", "X_cordin"), xm)
+
+
+@pytest.mark.gpu
+def test_fortran_mixed_precision_run(tmp_path):
+    """afid_run --mixed: the mixed-precision mode from the Fortran host; its final
+    fields stay within the stated mixed-vs-FP64 error of the FP64 run."""
+    need_run()
+    out = {}
+    for mode in ("fp64", "mixed"):
+        d = tmp_path / mode
+        d.mkdir()
+        write_bou(d / "bou.in", ntst=3)
+        cmd = [RUN, "bou.in", "--seed", "4", "--out", "."] + (["--mixed"] if mode == "mixed" else [])
+        print("afid_run", " ".join(cmd[1:]), flush=True)
+        r = subprocess.run(cmd, cwd=d, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        assert "errorcode 555" in r.stdout
+        nx, nym, nzm = CASE["nxm"] + 1, CASE["nym"], CASE["nzm"]
+        out[mode] = {nm: np.fromfile(d / f"{nm}.bin", dtype=np.float64).reshape(nzm, nym, nx)
+                     for nm in ("vx", "vy", "vz", "temp")}
+    for nm in ("vx", "vy", "vz"):
+        assert rel_l2(out["mixed"][nm], out["fp64"][nm]) < 2e-5, nm
+    assert rel_l2(out["mixed"]["temp"], out["fp64"]["temp"]) < 1e-9
