@@ -299,3 +299,47 @@ def test_device_steps_stop_after_error_code(afid):
         assert rel_l2(dev.download(nm), host.download(nm)) < 1e-13, nm
     host.close()
     dev.close()
+
+
+# Sizes away from the small cases: column-solve instances for larger chunk
+# counts (L = 25, 44, 69, 80 -> LM 25, 48, 80, 80), x pitches above 512
+# positions (several positions per thread in the correction and reduction
+# kernels, several parts in the stencil pass), a y line of 768 (half length
+# 384: LDS above 64 KiB) and a z line of 750 (radices 2, 3, 5, 5, 5).
+EDGE_CASES = [
+    dict(nxm=384, nym=8, nzm=6, istr3=6, str3=6.0, ylen=1.0, zlen=1.0, ray=1e6, pra=1.0),
+    dict(nxm=700, nym=6, nzm=10, istr3=6, str3=6.0, ylen=1.0, zlen=1.0, ray=1e6, pra=1.0),
+    dict(nxm=1100, nym=4, nzm=6, istr3=6, str3=6.0, ylen=1.0, zlen=1.0, ray=1e6, pra=1.0),
+    dict(nxm=1279, nym=4, nzm=4, istr3=6, str3=6.0, ylen=1.0, zlen=1.0, ray=1e6, pra=1.0),
+    dict(nxm=12, nym=768, nzm=6, istr3=6, str3=6.0, ylen=1.0, zlen=1.0, ray=1e6, pra=1.0),
+    dict(nxm=10, nym=6, nzm=750, istr3=4, str3=3.0, ylen=1.0, zlen=1.0, ray=1e6, pra=1.0),
+]
+
+
+@pytest.mark.parametrize("c", EDGE_CASES, ids=case_id)
+def test_edge_sizes_match_oracle(afid, c):
+    """Four host-driven steps, and eight device steps from captured graphs, each
+    against the oracle at rel-L2 <= 1e-11; fixed dt 5e-4, inside the explicit
+    diffusion limit of the finest lateral grids here."""
+    o, cell = make_pair(afid, c)
+    o.init_linear_noise(5, 1e-3)
+    cell.init_linear_noise(5, 1e-3)
+    print("edge case", case_id(c), flush=True)
+    for n in range(4):
+        o.step(n, 0, 5e-4, 1e-4, 5e-3, 1.2)
+        cell.step(n, 0, 5e-4, 1e-4, 5e-3, 1.2)
+    nxm = o.nxm
+    for nm in ("vx", "vy", "vz", "temp"):
+        assert rel_l2(cell.download(nm), o.interior(nm)) < 1e-11, nm
+    assert rel_l2(demean(cell.download("pr")[:, :, :nxm]), demean(o.interior("pr")[:, :, :nxm])) < 1e-11
+    assert cell.CheckDivergence() < 1e-10
+    other = afid.Cell(afid.CellParams(nsst=3, alx3=1.0, **c))
+    other.init_linear_noise(5, 1e-3)
+    for n in range(4, 8):
+        o.step(n, 0, 5e-4, 1e-4, 5e-3, 1.2)
+    err, _, done = other.steps(0, 8, 0, 5e-4, 1e-4, 5e-3, 1.2)
+    assert err == 0 and done == 8
+    for nm in ("vx", "vy", "vz", "temp"):
+        assert rel_l2(other.download(nm), o.interior(nm)) < 1e-11, nm
+    cell.close()
+    other.close()
