@@ -92,3 +92,26 @@ def test_params_precision_field():
     import ctypes
     p = CellParams(nxm=8, nym=8, nzm=8, precision=PREC_MIXED).to_c()
     assert p.precision == PREC_MIXED and ctypes.sizeof(_Params) % 8 == 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("c", [dict(nxm=12, nym=768, nzm=6), dict(nxm=10, nym=6, nzm=750)], ids=["y768", "z750"])
+def test_mixed_long_fft_lines(c):
+    """The FP32 FFT instances on a y line of 768 and a z line of 750: mixed
+    against FP64 after 8 steps of dt 5e-4 (velocities < 1e-4, temp < 1e-8 rel-L2,
+    max|div| < 1e-8)."""
+    from afid_amd.engine import Cell, CellParams, PREC_FP64, PREC_MIXED
+    out = {}
+    for prec in (PREC_FP64, PREC_MIXED):
+        cell = Cell(CellParams(istr3=6, str3=6.0, ylen=1.0, zlen=1.0, ray=1e6, pra=1.0, precision=prec, **c))
+        cell.init_linear_noise(5, 1e-3)
+        err, _, done = cell.steps(0, 8, 0, 5e-4)
+        assert err == 0 and done == 8
+        out[prec] = {nm: cell.download(nm) for nm in ("vx", "vy", "vz", "temp")}
+        out[prec]["div"] = cell.CheckDivergence()
+        cell.close()
+    errs = {nm: rel_l2(out[PREC_MIXED][nm], out[PREC_FP64][nm]) for nm in ("vx", "vy", "vz", "temp")}
+    print("long-line mixed vs fp64", c, errs, out[PREC_MIXED]["div"], flush=True)
+    for nm in ("vx", "vy", "vz"):
+        assert errs[nm] < 1e-4, nm
+    assert errs["temp"] < 1e-8 and out[PREC_MIXED]["div"] < 1e-8
