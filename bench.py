@@ -242,8 +242,8 @@ def main():
     ap.add_argument("--requests", type=int, default=24)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(each cell runs on one device; see DESIGN.md for how the path would shard)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(each cell runs on one device and is not sharded across GPUs)", file=sys.stderr)
         sys.exit(2)
     if args.workload == "sweep":
         return bench_sweep(args)

@@ -5,7 +5,7 @@ Tolerances (FP64 everywhere):
   per routine           rel-L2 <= 1e-13 (same operation order up to FMA contraction)
   implicit solves       rel-L2 <= 1e-12 (pivot-free LU + lane scan vs netlib dgttrs)
   pressure solve        rel-L2 <= 1e-11 on dph modulo its mean (FFT and Thomas order differ)
-  10 composed steps     rel-L2 <= 1e-11 per field, pr modulo its mean
+  10 composed steps     rel-L2 <= 1e-11 per field, pr modulo its mean (the bar of the path)
 """
 import numpy as np
 import pytest
