@@ -83,6 +83,7 @@ struct afid_ctx {
   size_t bytes_held = 0;
   bool dev_steps_ready = false;   // device step state set and the step graph captured
   bool mixed = false;             // AFID_PREC_MIXED: FP32 histories and FFT spectrum
+  int zm = 0;                     // z planes per stencil block chosen by afid_tune (0: default)
   DevPlan dpy, dpz;
   RealPlan py{};
   FFTPlan pz{};
@@ -1069,8 +1070,7 @@ int afid_update_halo(afid_ctx *c, int field) {
 // One substep's kernels on the context's stream.  dev = false: coefficients
 // and factor rows from the host (afid_set_substep, upload_factors); dev = true:
 // both come from the device step state (afid_steps), nothing is copied.
-static int enqueue_substep(afid_ctx *c, int ns, bool dev) {
-  int rc;
+static FusedArgs update_args(afid_ctx *c, int ns, bool dev) {
   FusedArgs a;
   a.g = c->g;
   a.f = fields(c);
@@ -1089,6 +1089,13 @@ static int enqueue_substep(afid_ctx *c, int ns, bool dev) {
   a.ds = dev ? c->dstep : nullptr;
   a.ns = ns;
   a.h32 = c->mixed;
+  a.zm = c->zm;
+  return a;
+}
+
+static int enqueue_substep(afid_ctx *c, int ns, bool dev) {
+  int rc;
+  FusedArgs a = update_args(c, ns, dev);
   if (c->split) {
     {
       Timed t(c, "stencil_rhs");
@@ -1340,6 +1347,56 @@ int afid_batch_steps(afid_ctx *const *cells, int ncells, int nsteps) {
       int rc = dev_steps_enqueue(cells[q], 1);
       if (rc) return rc;
     }
+  return AFID_OK;
+}
+
+// Autotuning of the stencil pass's z planes per block: each candidate is
+// timed with HIP events on this context's stream (one warm-up and three timed
+// launches of the substep-2 form, which reads the histories); the fastest is
+// kept.  The launches write the histories and the b' scratch, which are
+// cleared afterwards, so the state is to be set after this call.
+int afid_tune(afid_ctx *c, int *zm_out) {
+  if (!c) return AFID_ERR_ARG;
+  if (!c->split) {   // only the split update has the stencil pass
+    if (zm_out) *zm_out = 0;
+    return AFID_OK;
+  }
+  static const int cands[] = {8, 16, 32, 64};
+  hipEvent_t e0, e1;
+  HIPCHK(c, hipEventCreate(&e0));
+  HIPCHK(c, hipEventCreate(&e1));
+  FusedArgs a = update_args(c, 2, false);
+  a.sc.use_ru = 1;
+  int best = 0, rc = AFID_OK;
+  float best_ms = 0.f;
+  for (int zm : cands) {
+    if (zm > c->g.nzm && zm != cands[0]) break;   // a larger block than the grid adds nothing
+    a.zm = zm;
+    if ((rc = kerr(c, launch_stencil_rhs(a, c->st)))) break;
+    if ((rc = kerr(c, hipEventRecord(e0, c->st)))) break;
+    for (int r = 0; r < 3 && !rc; r++) rc = kerr(c, launch_stencil_rhs(a, c->st));
+    if (rc || (rc = kerr(c, hipEventRecord(e1, c->st))) || (rc = kerr(c, hipEventSynchronize(e1)))) break;
+    float ms = 0.f;
+    if ((rc = kerr(c, hipEventElapsedTime(&ms, e0, e1)))) break;
+    if (best == 0 || ms < best_ms) {
+      best = zm;
+      best_ms = ms;
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc) return rc;
+  for (int q : {AFID_RUX, AFID_RUY, AFID_RUZ, AFID_RUTEMP})
+    HIPCHK(c, hipMemsetAsync(c->f[q], 0, field_bytes(c), c->st));
+  for (int q = 0; q < 4; q++) HIPCHK(c, hipMemsetAsync(c->bp[q], 0, field_bytes(c), c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  c->zm = best;
+  if (c->step_graph) {   // a captured step holds the previous launch shape
+    hipGraphExecDestroy(c->step_graph);
+    c->step_graph = nullptr;
+    c->dev_steps_ready = false;
+  }
+  if (zm_out) *zm_out = best;
   return AFID_OK;
 }
 

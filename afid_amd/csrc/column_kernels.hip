@@ -548,7 +548,7 @@ __global__ void __launch_bounds__(WG)
 // b' = ackl_b*rhs + A q_old, where A is the unit-diagonal normalised system of
 // SolveImpEqnUpdate_*; identity rows store q_old (+ rhs(1) = the stale row for
 // vx).  A block owns up to 256 stored positions of one y column j and marches
-// through ZM consecutive z planes: each thread keeps its position's values of
+// through zplanes consecutive z planes: each thread keeps its position's values of
 // planes i-1, i and i+1 in registers, so the z neighbours cost no loads and
 // each field value is fetched from L2 once for the centre column and once for
 // each y neighbour.  Blocks are ordered z-range major, then y (XCD-remapped),
@@ -562,7 +562,7 @@ __global__ void __launch_bounds__(WG)
 // The vz increment at k=1, q_new - q_old, becomes the new stale row and the
 // temperature's rhs(1).
 constexpr int STENCIL_WG = 256;
-constexpr int ZM = 32;   // z planes per block
+constexpr int ZM_DEFAULT = 32;   // z planes per block unless afid_tune chose another count
 // Coefficient rows staged in LDS by k_stencil_rhs: eight per-k metric rows,
 // then scale, dl, du of the x, y/z and temperature systems.
 enum StencilCoef {
@@ -608,7 +608,8 @@ __global__ void __launch_bounds__(STENCIL_WG)
   const Geom g = a.g;
   const int L = g.L, ldx = g.ldx, nx = g.nx, nxm = g.nxm;
   const int nparts = (ldx + STENCIL_WG - 1) / STENCIL_WG;
-  const int nib = (g.nzm + ZM - 1) / ZM;
+  const int zplanes = a.zm > 0 ? a.zm : ZM_DEFAULT;
+  const int nib = (g.nzm + zplanes - 1) / zplanes;
   const int64_t tile = xcd_block((int64_t)nib * g.nym * nparts);
   if (tile < 0) return;
   const int part = (int)(tile % nparts);
@@ -640,7 +641,7 @@ __global__ void __launch_bounds__(STENCIL_WG)
   const double udx3c = cfv(SC_UDX3C), udx3m = cfv(SC_UDX3M);
   const double amc = cfv(SC_AMC), acc = cfv(SC_ACC), apc = cfv(SC_APC);
   const double ams = cfv(SC_AMS), acs = cfv(SC_ACS), aps = cfv(SC_APS);
-  const int i0 = ib * ZM, i1 = min(i0 + ZM, g.nzm);
+  const int i0 = ib * zplanes, i1 = min(i0 + zplanes, g.nzm);
   // prologue: plane i0 and the values of plane i0-1 the marching step needs
   Plane cur = load_plane<OffT>(vx, vy, vz, T, pr, colb(jc, i0), smb, sb, spb);
   const int ib0 = wrapm(i0 - 1, g.nzm);
@@ -1150,7 +1151,8 @@ hipError_t launch_fused_update(const FusedArgs &a, hipStream_t st) {
 
 hipError_t launch_stencil_rhs(const FusedArgs &a, hipStream_t st) {
   const int64_t n = a.g.ncol * a.g.ldx;
-  const int64_t ntile = (int64_t)((a.g.nzm + ZM - 1) / ZM) * a.g.nym * ((a.g.ldx + STENCIL_WG - 1) / STENCIL_WG);
+  const int zplanes = a.zm > 0 ? a.zm : ZM_DEFAULT;
+  const int64_t ntile = (int64_t)((a.g.nzm + zplanes - 1) / zplanes) * a.g.nym * ((a.g.ldx + STENCIL_WG - 1) / STENCIL_WG);
   const dim3 grid((unsigned)(((ntile + 7) / 8) * 8));
   const size_t shm = stencil_lds(a.g);
   const bool small = (uint64_t)n * sizeof(double) < (1ull << 32);

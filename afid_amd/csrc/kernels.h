@@ -17,6 +17,7 @@ struct FusedArgs {
   const DevStep *ds;                 // non-null: scalars of substep ns come from the device
   int ns;
   int h32;                           // histories stored in FP32 (mixed precision)
+  int zm;                            // z planes per stencil block (0: the default)
 };
 
 struct RoutineArgs {

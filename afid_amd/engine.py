@@ -87,6 +87,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "afid_steps_wait": ([vp, pd, ctypes.POINTER(i), ctypes.POINTER(i)], i),
         "afid_batch_steps": ([ctypes.POINTER(vp), i, i], i),
         "afid_estimate_bytes": ([ctypes.c_void_p, ctypes.POINTER(i64)], i),
+        "afid_tune": ([vp, ctypes.POINTER(i)], i),
         "afid_get_time": ([vp, pd], i),
         "afid_set_time": ([vp, d], i),
         "afid_get_stale": ([vp, pd], i),
@@ -376,6 +377,13 @@ class Cell:
         """Queues nsteps device steps on this cell's stream and returns at once."""
         self._check(self.L.afid_steps_async(self.h, ntime0, nsteps, idtv, dt, dtmin, dtmax, limitcfl),
                     "steps_async")
+
+    def tune(self) -> int:
+        """Autotunes the stencil pass's z planes per block on this cell (call before
+        setting the initial state); returns the choice (0: fused update, untuned)."""
+        zm = ctypes.c_int()
+        self._check(self.L.afid_tune(self.h, ctypes.byref(zm)), "tune")
+        return zm.value
 
     def steps_more(self, nsteps: int):
         self._check(self.L.afid_steps_more(self.h, nsteps), "steps_more")

@@ -339,6 +339,12 @@
           type(c_ptr), intent(in) :: cells(*)
           integer(c_int), value :: ncells, nsteps
         end function
+        integer(c_int) function afid_tune(ctx, zm)                      &
+     &      bind(C, name='afid_tune')
+          import :: c_int, c_ptr
+          type(c_ptr), value :: ctx
+          integer(c_int), intent(out) :: zm
+        end function
         integer(c_int) function afid_estimate_bytes(p, nbytes)          &
      &      bind(C, name='afid_estimate_bytes')
           import :: c_int, c_int64_t, afid_params_t

@@ -93,15 +93,15 @@ class _Active:
 class Server:
     """Continuous-batching request loop over resident cells.
 
-    make_cell(params) -> object with init_linear_noise, steps_async, steps_more,
-    busy, steps_wait, CheckDivergence, close (engine.Cell by default);
+    make_cell(params) -> object with tune, init_linear_noise, steps_async,
+    steps_more, busy, steps_wait, CheckDivergence, close (engine.Cell by default);
     estimate(params) -> bytes; on_finish(request, cell) runs before the cell is
     closed (the tests read the fields there)."""
 
     def __init__(self, budget_bytes: float = 200e9, chunk: int = 50,
                  make_cell: Callable = Cell, estimate: Callable = estimate_bytes,
                  clock: Callable[[], float] = time.perf_counter, idle_sleep: float = 2e-4,
-                 on_finish: Optional[Callable[[Request, object], None]] = None):
+                 on_finish: Optional[Callable[[Request, object], None]] = None, tune: bool = True):
         if chunk < 1:
             raise ValueError("chunk must be at least 1 step")
         self.budget = budget_bytes
@@ -111,6 +111,7 @@ class Server:
         self.clock = clock
         self.idle_sleep = idle_sleep
         self.on_finish = on_finish
+        self.tune = tune                # each admitted cell's stencil launch shape from afid_tune
         self.peak_bytes = 0
         self.peak_active = 0
 
@@ -138,6 +139,8 @@ class Server:
                 cell = None
                 try:
                     cell = self.make_cell(r.spec.params)
+                    if self.tune:
+                        cell.tune()
                     cell.init_linear_noise(r.spec.seed, 1e-3)
                     first = min(self.chunk, r.spec.steps)
                     cell.steps_async(0, first, ctl["idtv"], ctl["dt"], ctl["dtmin"], ctl["dtmax"],

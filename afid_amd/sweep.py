@@ -79,13 +79,17 @@ class SweepResult:
                 "all_steps_done": self.steps_done == self.steps_wanted}
 
 
-def run_batched(specs: List[CellSpec], on_done: Optional[Callable[[int, Cell], None]] = None) -> SweepResult:
-    """All cells resident at once; steps queued interleaved across the cells."""
+def run_batched(specs: List[CellSpec], on_done: Optional[Callable[[int, Cell], None]] = None,
+                tune: bool = True) -> SweepResult:
+    """All cells resident at once; steps queued interleaved across the cells.
+    tune: each cell's stencil launch shape is picked by afid_tune first."""
     t0 = time.perf_counter()
     cells = [Cell(s.params) for s in specs]
     try:
         ctl = STEP_CTL
         for s, c in zip(specs, cells):
+            if tune:
+                c.tune()
             c.init_linear_noise(s.seed, 1e-3)
             c.steps_async(0, 0, ctl["idtv"], ctl["dt"], ctl["dtmin"], ctl["dtmax"], ctl["limitcfl"])
         # equal step counts (the sweep's case) go in one interleaved call;

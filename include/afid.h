@@ -178,6 +178,13 @@ int afid_steps_wait(afid_ctx *ctx, double *dt_last, int *errorcode, int *steps_d
 /* Queues nsteps more steps on each of ncells contexts, step n of every cell
  * before step n+1 of any; each cell must have had afid_steps_async. */
 int afid_batch_steps(afid_ctx *const *cells, int ncells, int nsteps);
+/* Autotuner for the stencil pass (split update): times z planes per block in
+ * {8, 16, 32, 64} (not above nzm) on this context and keeps the fastest for
+ * its later launches; *zm receives the choice (0 with AFID_UPDATE=fused).  It
+ * writes the histories and scratch while timing and clears them afterwards:
+ * call it before setting the initial state.  No counterpart in the
+ * reference. */
+int afid_tune(afid_ctx *ctx, int *zm);
 /* Device bytes a context for p holds at most (fields, spectra, factor rows,
  * device step state, staging, statistics, nine slab planes); for admission
  * by memory budget. */

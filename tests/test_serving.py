@@ -59,6 +59,9 @@ class FakeCell:
         FakeCell.live += 1
         FakeCell.peak = max(FakeCell.peak, FakeCell.live)
 
+    def tune(self):
+        return 16
+
     def init_linear_noise(self, seed, amp):
         self.seed = seed
 
@@ -228,3 +231,20 @@ def test_server_records_refused_request():
     assert by[0].steps_done == by[2].steps_done == 20 and FakeCell.live == 0
     s = summarize(done)
     assert s["requests"] == 3 and s["refused"] == 1
+
+
+@pytest.mark.gpu
+def test_tuned_cell_equals_untuned():
+    """afid_tune changes only the launch shape of the stencil pass: a tuned
+    cell's fields equal an untuned one's bit for bit."""
+    from afid_amd.engine import Cell
+    spec = _spec(0, 6)
+    want, _, _ = _alone(spec)
+    c = Cell(spec.params)
+    zm = c.tune()
+    assert zm in (8, 16, 32, 64)
+    c.init_linear_noise(spec.seed, 1e-3)
+    c.steps(0, spec.steps, **{k: STEP_CTL[k] for k in ("idtv", "dt", "dtmin", "dtmax", "limitcfl")})
+    for nm in ("vx", "vy", "vz", "temp", "pr"):
+        np.testing.assert_array_equal(c.download(nm), want[nm], err_msg=nm)
+    c.close()
