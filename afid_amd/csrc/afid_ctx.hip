@@ -1630,7 +1630,7 @@ int afid_interp_field(afid_ctx *c, int field, const double *old, int nxo, int ny
   }
   const Geom &g = c->g;
   const int nx = g.nx, nym = g.nym, nzm = g.nzm, ny = nym + 1, nz = nzm + 1;
-  if (nxo < 2 || nyo < 3 || nzo < 3) return AFID_ERR_ARG;
+  if (nxo < 2 || nyo < 3 || nzo < 3 || nxo > 65536 || nyo > 65536 || nzo > 65536) return AFID_ERR_ARG;
   if (nz > nzo * 2 || ny > nyo * 2 || nx > nxo * 2) return AFID_ERR_GRID;   // ReadFlowField.F90:72-82
   const int nyom = nyo - 1, nzom = nzo - 1;
   // one-rank window of ReadFlowField.F90:91-99 (integer divisions)

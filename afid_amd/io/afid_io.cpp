@@ -9,6 +9,7 @@
 #include <sys/stat.h>
 
 #include <cmath>
+#include <new>
 #include <cstdio>
 #include <string>
 #include <vector>
@@ -300,8 +301,16 @@ int afid_read_continuation(afid_ctx *ctx, const char *dir) {
     // another grid: read each field on its own grid and interpolate
     // (ReadFlowField.F90:64-158); pr, the histories and the stale row start
     // from zero, as the reference restarts
-    if (nxo < 2 || nyo < 3 || nzo < 3) return AFID_ERR_IO;
-    std::vector<double> o((size_t)nxo * (nyo - 1) * (nzo - 1));
+    // sizes from the file: refuse implausible ones before allocating
+    if (nxo < 2 || nyo < 3 || nzo < 3 || nxo > 65536 || nyo > 65536 || nzo > 65536 ||
+        (double)nxo * nyo * nzo > 6.9e10)
+      return AFID_ERR_IO;
+    std::vector<double> o;
+    try {
+      o.resize((size_t)nxo * (nyo - 1) * (nzo - 1));
+    } catch (const std::bad_alloc &) {
+      return AFID_ERR_NOMEM;
+    }
     for (const FieldFile &ff : kReferenceFields) {
       if ((rc = read_field_file(join(dir, ff.file), ff.dset, o.data(), nxo, nyo - 1, nzo - 1))) return rc;
       if ((rc = afid_interp_field(ctx, ff.field, o.data(), nxo, nyo, nzo, istro3, stro3))) return rc;
