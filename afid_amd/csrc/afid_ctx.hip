@@ -1022,6 +1022,7 @@ int afid_solve_pressure_correction(afid_ctx *c) {
 
 int afid_pressure_spectrum(afid_ctx *c, int stage, double *out) {
   if (!c || !out || stage < 1 || stage > 4) return AFID_ERR_ARG;
+  if (c->mixed) return AFID_ERR_STATE;   // the spectrum is FP32 in that mode
   PoissonArgs a = poisson_args(c, 0);
   int rc;
   if ((rc = kerr(c, launch_yfft_forward(a, c->st)))) return rc;

@@ -133,7 +133,8 @@ int afid_update_halo(afid_ctx *ctx, int field);   /* halo.F90 update_halo: no-op
  * stage 1 (y r2c FFT, :98), 2 (+ z forward FFT, :102), 3 (+ x tridiagonal
  * solves incl. the 1/(nym nzm) scaling, :105-141) or 4 (+ z backward FFT,
  * :145) and copies the spectrum cz1(k, m, n) into out as interleaved complex,
- * Fortran order (nxm, nym/2+1, nzm).  dph itself is left unchanged. */
+ * Fortran order (nxm, nym/2+1, nzm).  dph itself is left unchanged.
+ * AFID_ERR_STATE in the mixed-precision mode (FP32 spectrum). */
 int afid_pressure_spectrum(afid_ctx *ctx, int stage, double *out);
 
 /* ---- fused time advance (TimeMarcher.F90:13-78) -----------------------------
