@@ -547,22 +547,25 @@ __global__ void __launch_bounds__(WG)
 // (ExplicitTerms*, ImplicitAndUpdate* RHS loops), the history update, and
 // b' = ackl_b*rhs + A q_old, where A is the unit-diagonal normalised system of
 // SolveImpEqnUpdate_*; identity rows store q_old (+ rhs(1) = the stale row for
-// vx).  A block owns up to 256 stored positions of one y column j and marches
-// through zplanes consecutive z planes: each thread keeps its position's values of
-// planes i-1, i and i+1 in registers, so the z neighbours cost no loads and
-// each field value is fetched from L2 once for the centre column and once for
-// each y neighbour.  Blocks are ordered z-range major, then y (XCD-remapped),
-// so the y neighbour columns are being read by neighbouring blocks of the same
-// XCD at the same time.  Every store is unconditional (selects instead of
-// branches) so that a plane's loads issue in one batch; the history rows
-// outside the reference loops (k = 1 and nx for vx/temp, nx for vy/vz), which
-// nothing reads, are kept at 0.
+// vx).  A block has SY_COLS waves on SY_COLS neighbouring y columns, all at
+// the same 64 stored positions, and marches through zplanes consecutive z
+// planes: each thread keeps its position's values of planes i-1, i and i+1 in
+// registers, so the z neighbours cost no loads.  Each wave publishes the plane
+// values its y neighbours need in LDS (double-buffered by plane, one barrier
+// per plane), so only the group's two edge columns read a y neighbour from
+// memory.  Blocks are ordered position-range fastest, then y group, then z
+// range (XCD-remapped), so an edge column is being read by the neighbouring
+// group's block on the same XCD at the same time.  A plane's loads issue in
+// one batch before any value is used; the history rows outside the reference
+// loops (k = 1 and nx for vx/temp, nx for vy/vz), which nothing reads, are
+// kept at 0.
 // k_column_solve: a wave owns four columns (16 chunk lanes each) and solves the
 // four systems in registers: A q_new = b', with the factor rows staged in LDS.
 // The vz increment at k=1, q_new - q_old, becomes the new stale row and the
 // temperature's rhs(1).
-constexpr int STENCIL_WG = 256;
 constexpr int ZM_DEFAULT = 32;   // z planes per block unless afid_tune chose another count
+constexpr int SY_COLS = 4;       // y columns per block, one wave each
+constexpr int SY_WG = SY_COLS * 64;
 // Coefficient rows staged in LDS by k_stencil_rhs: eight per-k metric rows,
 // then scale, dl, du of the x, y/z and temperature systems.
 enum StencilCoef {
@@ -574,7 +577,6 @@ __device__ __forceinline__ int kc_row_of(int r) {
   return rows[r];
 }
 __device__ __forceinline__ int fac_row_of(int q) { return q == 0 ? FR_SCALE : (q == 1 ? FR_DL : FR_DU); }
-static size_t stencil_lds(const Geom &) { return sizeof(double) * SC_N * STENCIL_WG; }
 
 // The centre-column values of one plane at a thread's position (k-1, k, k+1).
 struct Plane {
@@ -593,150 +595,191 @@ __device__ __forceinline__ Plane load_plane(const double *__restrict__ vx, const
   return q;
 }
 
+// Plane values a wave publishes for its y neighbours (the k-1 value of vy and
+// the k+1 value of vx are those at the neighbour's own position s).
+enum { SV_VX = 0, SV_VY, SV_VYM, SV_VZ, SV_T, SV_VXP, SV_PR, SV_VZIP, SV_N };
+static size_t stencil_lds(const Geom &) { return sizeof(double) * (SC_N * 64 + 2 * SV_N * SY_WG); }
+
 // DEV: the substep scalars come from the device step state (a.ds) rather
 // than from the argument block; two instantiations keep each one's scalar
 // loads unconditional.
 template <typename OffT, bool USE_RU, bool DEV, bool H32>
-__global__ void __launch_bounds__(STENCIL_WG)
+__global__ void __launch_bounds__(SY_WG)
     k_stencil_rhs(FusedArgs a, const double *__restrict__ vx, const double *__restrict__ vy,
                   const double *__restrict__ vz, const double *__restrict__ T, const double *__restrict__ pr,
                   double *__restrict__ rux, double *__restrict__ ruy, double *__restrict__ ruz,
                   double *__restrict__ rut, double *__restrict__ bx, double *__restrict__ by,
                   double *__restrict__ bz, double *__restrict__ bt, const double *__restrict__ stalep,
                   const double *__restrict__ kc, const double *__restrict__ facp) {
-  extern __shared__ double cf[];   // [SC_N][STENCIL_WG] per-k coefficient rows of the block's positions
+  extern __shared__ double sh[];
+  double *cf = sh;                    // [SC_N][64] coefficient rows of the block's positions
+  double *xv = sh + SC_N * 64;        // [2][SV_N][SY_WG] published plane values
   const Geom g = a.g;
   const int L = g.L, ldx = g.ldx, nx = g.nx, nxm = g.nxm;
-  const int nparts = (ldx + STENCIL_WG - 1) / STENCIL_WG;
+  const int nq = (ldx + 63) / 64;
+  const int ngr = (g.nym + SY_COLS - 1) / SY_COLS;
   const int zplanes = a.zm > 0 ? a.zm : ZM_DEFAULT;
   const int nib = (g.nzm + zplanes - 1) / zplanes;
-  const int64_t tile = xcd_block((int64_t)nib * g.nym * nparts);
+  const int64_t tile = xcd_block((int64_t)nib * ngr * nq);
   if (tile < 0) return;
-  const int part = (int)(tile % nparts);
-  const int64_t rest = tile / nparts;
-  const int jc = (int)(rest % g.nym), ib = (int)(rest / g.nym);
+  const int q = (int)(tile % nq);
+  const int64_t rest = tile / nq;
+  const int gr = (int)(rest % ngr), ib = (int)(rest / ngr);
   if (DEV && a.ds->halted) return;
   const StepScalars c = DEV ? a.ds->sc[a.ns - 1] : a.sc;
-  const int s = part * STENCIL_WG + (int)threadIdx.x;
-  for (int idx = threadIdx.x; idx < SC_N * STENCIL_WG; idx += STENCIL_WG) {
-    const int r = idx / STENCIL_WG, sl = min(part * STENCIL_WG + idx - r * STENCIL_WG, ldx - 1);
+  for (int idx = threadIdx.x; idx < SC_N * 64; idx += SY_WG) {
+    const int r = idx / 64, sl = min(q * 64 + idx - r * 64, ldx - 1);
     const int src = r < SC_XSC ? kc_row_of(r) * ldx : (((r - SC_XSC) / 3) * FR_N + fac_row_of((r - SC_XSC) % 3)) * ldx;
     cf[idx] = (r < SC_XSC ? kc : facp)[src + sl];
   }
   __syncthreads();
-  if (s >= ldx) return;
+  const int lane = (int)threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  const int ncols = min(SY_COLS, g.nym - gr * SY_COLS);   // columns of this group
+  const bool colok = w < ncols;
+  const int jc = colok ? gr * SY_COLS + w : g.nym - 1;    // idle waves read a valid column, store nothing
+  const int sraw = q * 64 + lane;
+  const int s = min(sraw, ldx - 1);
   const int k = k_of_pos(s, L);
-  if (k >= nx) return;
+  const bool live = colok && sraw < ldx && k < nx;
+  const bool jp_in = w + 1 < ncols, jm_in = w > 0;        // neighbour column in this block
   const int jm = wrapm(jc - 1, g.nym), jp = wrapp(jc + 1, g.nym);
   const OffT cb = (OffT)ldx * sizeof(double);
   auto colb = [&](int jj, int ii) { return ((OffT)ii * (OffT)g.nym + (OffT)jj) * cb; };
   const OffT sb = (OffT)s * sizeof(double);
-  // k-1 / k+1 positions; at the column ends they point at k itself (values unused there)
   const OffT smb = (OffT)(k >= 1 ? pos_of_k(k - 1, L) : s) * sizeof(double);
   const OffT spb = (OffT)(k + 1 < nx ? pos_of_k(k + 1, L) : s) * sizeof(double);
-  const bool inx = k >= 1 && k <= nx - 2;   // interior rows of the nx systems
-  const bool iny = k <= nxm - 1;            // rows of the nxm systems
-  const int sl = (int)threadIdx.x;
-  auto cfv = [&](int r) { return cf[r * STENCIL_WG + sl]; };
+  const bool inx = k >= 1 && k <= nx - 2;
+  const bool iny = k <= nxm - 1;
+  auto cfv = [&](int r) { return cf[r * 64 + lane]; };
   const double udx3c = cfv(SC_UDX3C), udx3m = cfv(SC_UDX3M);
   const double amc = cfv(SC_AMC), acc = cfv(SC_ACC), apc = cfv(SC_APC);
   const double ams = cfv(SC_AMS), acs = cfv(SC_ACS), aps = cfv(SC_APS);
   const int i0 = ib * zplanes, i1 = min(i0 + zplanes, g.nzm);
-  // prologue: plane i0 and the values of plane i0-1 the marching step needs
   Plane cur = load_plane<OffT>(vx, vy, vz, T, pr, colb(jc, i0), smb, sb, spb);
   const int ib0 = wrapm(i0 - 1, g.nzm);
   const OffT CP = colb(jc, ib0);
   double vxim = ldf(vx, CP + sb), vximp = ldf(vx, CP + spb), vyim = ldf(vy, CP + sb);
   double vzim = ldf(vz, CP + sb), tim = ldf(T, CP + sb), prim = ldf(pr, CP + sb);
   double vyjpim = ldf(vy, colb(jp, ib0) + sb);
+  const int tid = (int)threadIdx.x;
+  int par = 0;
   for (int ic = i0; ic < i1; ic++) {
     const int ip = wrapp(ic + 1, g.nzm);
     const OffT C = colb(jc, ic), CJM = colb(jm, ic), CJP = colb(jp, ic);
     const Plane nxt = load_plane<OffT>(vx, vy, vz, T, pr, colb(jc, ip), smb, sb, spb);
-    const double vxjp = ldf(vx, CJP + sb), vyjp = ldf(vy, CJP + sb), vyjpm = ldf(vy, CJP + smb);
-    const double vzjp = ldf(vz, CJP + sb), tjp = ldf(T, CJP + sb);
-    const double vxjm = ldf(vx, CJM + sb), vxjmp = ldf(vx, CJM + spb), vyjm = ldf(vy, CJM + sb);
-    const double vzjm = ldf(vz, CJM + sb), tjm = ldf(T, CJM + sb), prjm = ldf(pr, CJM + sb);
-    const double vzjmip = ldf(vz, colb(jm, ip) + sb);
+    double vxjp = 0.0, vyjp = 0.0, vyjpm = 0.0, vzjp = 0.0, tjp = 0.0;
+    double vxjm = 0.0, vxjmp = 0.0, vyjm = 0.0, vzjm = 0.0, tjm = 0.0, prjm = 0.0, vzjmip = 0.0;
+    if (!jp_in) {
+      vxjp = ldf(vx, CJP + sb); vyjp = ldf(vy, CJP + sb); vyjpm = ldf(vy, CJP + smb);
+      vzjp = ldf(vz, CJP + sb); tjp = ldf(T, CJP + sb);
+    }
+    if (!jm_in) {
+      vxjm = ldf(vx, CJM + sb); vxjmp = ldf(vx, CJM + spb); vyjm = ldf(vy, CJM + sb);
+      vzjm = ldf(vz, CJM + sb); tjm = ldf(T, CJM + sb); prjm = ldf(pr, CJM + sb);
+      vzjmip = ldf(vz, colb(jm, ip) + sb);
+    }
     const double rxo = USE_RU ? ldh<H32>(rux, C + sb) : 0.0, ryo = USE_RU ? ldh<H32>(ruy, C + sb) : 0.0;
     const double rzo = USE_RU ? ldh<H32>(ruz, C + sb) : 0.0, rto = USE_RU ? ldh<H32>(rut, C + sb) : 0.0;
-    const double st0 = stalep[(int64_t)ic * g.nym + jc];   // uniform address
+    const double st0 = stalep[(int64_t)ic * g.nym + jc];
     const double stale0 = k == 0 ? st0 : 0.0;
+    double *buf = xv + par * (SV_N * SY_WG);
+    buf[SV_VX * SY_WG + tid] = cur.vxc;
+    buf[SV_VY * SY_WG + tid] = cur.vyc;
+    buf[SV_VYM * SY_WG + tid] = cur.vym;
+    buf[SV_VZ * SY_WG + tid] = cur.vzc;
+    buf[SV_T * SY_WG + tid] = cur.tc;
+    buf[SV_VXP * SY_WG + tid] = cur.vxp;
+    buf[SV_PR * SY_WG + tid] = cur.prc;
+    buf[SV_VZIP * SY_WG + tid] = nxt.vzc;
+    __syncthreads();
+    if (jp_in) {
+      const int o = tid + 64;
+      vxjp = buf[SV_VX * SY_WG + o]; vyjp = buf[SV_VY * SY_WG + o]; vyjpm = buf[SV_VYM * SY_WG + o];
+      vzjp = buf[SV_VZ * SY_WG + o]; tjp = buf[SV_T * SY_WG + o];
+    }
+    if (jm_in) {
+      const int o = tid - 64;
+      vxjm = buf[SV_VX * SY_WG + o]; vxjmp = buf[SV_VXP * SY_WG + o]; vyjm = buf[SV_VY * SY_WG + o];
+      vzjm = buf[SV_VZ * SY_WG + o]; tjm = buf[SV_T * SY_WG + o]; prjm = buf[SV_PR * SY_WG + o];
+      vzjmip = buf[SV_VZIP * SY_WG + o];
+    }
+    par ^= 1;
     const double vxm = cur.vxm, vxc = cur.vxc, vxp = cur.vxp, vym = cur.vym, vyc = cur.vyc, vyp = cur.vyp;
     const double vzm = cur.vzm, vzc = cur.vzc, vzp = cur.vzp, tm = cur.tm, tc = cur.tc, tp = cur.tp;
     const double prm = cur.prm, prc = cur.prc;
     const double vxip = nxt.vxc, vyip = nxt.vyc, vzip = nxt.vzc, vzipm = nxt.vzm, tip = nxt.tc;
-
-    // ---- vx (ExplicitTermsVX.F90:55-101, ImplicitAndUpdateVX.F90:47-60)
-    {
-      double hxz = (((vzip + vzipm) * (vxip + vxc)) - ((vzc + vzm) * (vxc + vxim))) * c.udz4;
-      double hxy = (((vyjp + vyjpm) * (vxjp + vxc)) - ((vyc + vym) * (vxc + vxjm))) * c.udy4;
-      double aa = vxp + vxc, bb = vxc + vxm;
-      double hxx = (aa * aa - bb * bb) * udx3c * 0.25;
-      double dzzvx = (vxim - 2.0 * vxc + vxip) * c.udzq_re;
-      double dyyvx = (vxjm - 2.0 * vxc + vxjp) * c.udyq_re;
-      double q = -(hxx + hxy + hxz) + dyyvx + dzzvx + tc;
-      double dxxvx = vxp * apc + vxc * acc + vxm * amc;
-      double dxp = (prc - prm) * (c.al * udx3c);
-      double r = USE_RU ? c.ro * rxo : 0.0;
-      double rhs = (c.ga * q + r + c.alre * dxxvx - dxp) * c.dt;
-      double vin = rhs * cfv(SC_XSC) + ((cfv(SC_XDL) * vxm + vxc) + cfv(SC_XDU) * vxp);
-      double vbd = stale0 + vxc;
-      sth<H32>(rux, C + sb, inx ? q : 0.0);
-      stf(bx, C + sb, inx ? vin : vbd);
+    if (live) {
+      // ---- vx (ExplicitTermsVX.F90:55-101, ImplicitAndUpdateVX.F90:47-60)
+      {
+        double hxz = (((vzip + vzipm) * (vxip + vxc)) - ((vzc + vzm) * (vxc + vxim))) * c.udz4;
+        double hxy = (((vyjp + vyjpm) * (vxjp + vxc)) - ((vyc + vym) * (vxc + vxjm))) * c.udy4;
+        double aa = vxp + vxc, bb = vxc + vxm;
+        double hxx = (aa * aa - bb * bb) * udx3c * 0.25;
+        double dzzvx = (vxim - 2.0 * vxc + vxip) * c.udzq_re;
+        double dyyvx = (vxjm - 2.0 * vxc + vxjp) * c.udyq_re;
+        double q = -(hxx + hxy + hxz) + dyyvx + dzzvx + tc;
+        double dxxvx = vxp * apc + vxc * acc + vxm * amc;
+        double dxp = (prc - prm) * (c.al * udx3c);
+        double r = USE_RU ? c.ro * rxo : 0.0;
+        double rhs = (c.ga * q + r + c.alre * dxxvx - dxp) * c.dt;
+        double vin = rhs * cfv(SC_XSC) + ((cfv(SC_XDL) * vxm + vxc) + cfv(SC_XDU) * vxp);
+        double vbd = stale0 + vxc;
+        sth<H32>(rux, C + sb, inx ? q : 0.0);
+        stf(bx, C + sb, inx ? vin : vbd);
+      }
+      // ---- vy (ExplicitTermsVY.F90:52-98, ImplicitAndUpdateVY.F90:52-68); kmv/kpv clamps
+      {
+        const double vymm = k == 0 ? vyc : vym, vypp = k == nxm - 1 ? vyc : vyp;
+        double hyx = ((vxp + vxjmp) * (vypp + vyc) - (vxc + vxjm) * (vyc + vymm)) * udx3m * 0.25;
+        double yp = vyjp + vyc, ym = vyjm + vyc;
+        double hyy = (yp * yp - ym * ym) * c.udy4;
+        double hyz = ((vyip + vyc) * (vzip + vzjmip) - (vyc + vyim) * (vzc + vzjm)) * c.udz4;
+        double dyyvy = (vyjp - 2.0 * vyc + vyjm) * c.udyq_re;
+        double dzzvy = (vyip - 2.0 * vyc + vyim) * c.udzq_re;
+        double h = -(hyx + hyy + hyz) + dyyvy + dzzvy;
+        double dxxvy = vypp * aps + vyc * acs + vymm * ams;
+        double dyp = (prc - prjm) * c.udy_al;
+        double r = USE_RU ? c.ro * ryo : 0.0;
+        double rhs = (c.ga * h + r + c.alre * dxxvy - dyp) * c.dt;
+        double vin = rhs * cfv(SC_YSC) + ((cfv(SC_YDL) * vym + vyc) + cfv(SC_YDU) * vyp);
+        sth<H32>(ruy, C + sb, iny ? h : 0.0);
+        stf(by, C + sb, vin);   // row nx-1 is outside the nxm system and never read
+      }
+      // ---- vz (ExplicitTermsVZ.F90:56-101, ImplicitAndUpdateVZ.F90:48-64)
+      {
+        const double vzmm = k == 0 ? vzc : vzm, vzpp = k == nxm - 1 ? vzc : vzp;
+        double zp = vzip + vzc, zm = vzim + vzc;
+        double hzz = (zp * zp - zm * zm) * c.udz4;
+        double hzy = ((vyjp + vyjpim) * (vzjp + vzc) - (vyc + vyim) * (vzc + vzjm)) * c.udy4;
+        double hzx = ((vxp + vximp) * (vzpp + vzc) - (vxc + vxim) * (vzc + vzmm)) * udx3m * 0.25;
+        double dzzvz = (vzip - 2.0 * vzc + vzim) * c.udzq_re;
+        double dyyvz = (vzjp - 2.0 * vzc + vzjm) * c.udyq_re;
+        double h = -(hzx + hzy + hzz) + dyyvz + dzzvz;
+        double dxxvz = vzpp * aps + vzc * acs + vzmm * ams;
+        double dzp = (prc - prim) * c.dz * c.al;
+        double r = USE_RU ? c.ro * rzo : 0.0;
+        double rhs = (c.ga * h + r + c.alre * dxxvz - dzp) * c.dt;
+        double vin = rhs * cfv(SC_YSC) + ((cfv(SC_YDL) * vzm + vzc) + cfv(SC_YDU) * vzp);
+        sth<H32>(ruz, C + sb, iny ? h : 0.0);
+        stf(bz, C + sb, vin);
+      }
+      // ---- temperature (ExplicitTermsTemp.F90:52-93, ImplicitAndUpdateTemp.F90:41-54)
+      {
+        double htz = ((vzipm + vzip) * (tip + tc) - (vzm + vzc) * (tc + tim)) * c.udz4;
+        double hty = ((vyjp + vyjpm) * (tjp + tc) - (vyc + vym) * (tc + tjm)) * c.udy4;
+        double htx = ((vxp + vxc) * (tp + tc) - (vxc + vxm) * (tc + tm)) * udx3c * 0.25;
+        double dzzt = (tip - 2.0 * tc + tim) * c.udzq_pe;
+        double dyyt = (tjp - 2.0 * tc + tjm) * c.udyq_pe;
+        double h = -(htx + hty + htz) + dyyt + dzzt;
+        double dxxt = tp * apc + tc * acc + tm * amc;
+        double r = USE_RU ? c.ro * rto : 0.0;
+        double rhs = (c.ga * h + r + c.alpec * dxxt) * c.dt;
+        double vin = rhs * cfv(SC_TSC) + ((cfv(SC_TDL) * tm + tc) + cfv(SC_TDU) * tp);
+        sth<H32>(rut, C + sb, inx ? h : 0.0);
+        stf(bt, C + sb, inx ? vin : tc);   // row 1 gets the vz increment added by the solve pass
+      }
     }
-    // ---- vy (ExplicitTermsVY.F90:52-98, ImplicitAndUpdateVY.F90:52-68); kmv/kpv clamps
-    {
-      const double vymm = k == 0 ? vyc : vym, vypp = k == nxm - 1 ? vyc : vyp;
-      double hyx = ((vxp + vxjmp) * (vypp + vyc) - (vxc + vxjm) * (vyc + vymm)) * udx3m * 0.25;
-      double yp = vyjp + vyc, ym = vyjm + vyc;
-      double hyy = (yp * yp - ym * ym) * c.udy4;
-      double hyz = ((vyip + vyc) * (vzip + vzjmip) - (vyc + vyim) * (vzc + vzjm)) * c.udz4;
-      double dyyvy = (vyjp - 2.0 * vyc + vyjm) * c.udyq_re;
-      double dzzvy = (vyip - 2.0 * vyc + vyim) * c.udzq_re;
-      double h = -(hyx + hyy + hyz) + dyyvy + dzzvy;
-      double dxxvy = vypp * aps + vyc * acs + vymm * ams;
-      double dyp = (prc - prjm) * c.udy_al;
-      double r = USE_RU ? c.ro * ryo : 0.0;
-      double rhs = (c.ga * h + r + c.alre * dxxvy - dyp) * c.dt;
-      double vin = rhs * cfv(SC_YSC) + ((cfv(SC_YDL) * vym + vyc) + cfv(SC_YDU) * vyp);
-      sth<H32>(ruy, C + sb, iny ? h : 0.0);
-      stf(by, C + sb, vin);   // row nx-1 is outside the nxm system and never read
-    }
-    // ---- vz (ExplicitTermsVZ.F90:56-101, ImplicitAndUpdateVZ.F90:48-64)
-    {
-      const double vzmm = k == 0 ? vzc : vzm, vzpp = k == nxm - 1 ? vzc : vzp;
-      double zp = vzip + vzc, zm = vzim + vzc;
-      double hzz = (zp * zp - zm * zm) * c.udz4;
-      double hzy = ((vyjp + vyjpim) * (vzjp + vzc) - (vyc + vyim) * (vzc + vzjm)) * c.udy4;
-      double hzx = ((vxp + vximp) * (vzpp + vzc) - (vxc + vxim) * (vzc + vzmm)) * udx3m * 0.25;
-      double dzzvz = (vzip - 2.0 * vzc + vzim) * c.udzq_re;
-      double dyyvz = (vzjp - 2.0 * vzc + vzjm) * c.udyq_re;
-      double h = -(hzx + hzy + hzz) + dyyvz + dzzvz;
-      double dxxvz = vzpp * aps + vzc * acs + vzmm * ams;
-      double dzp = (prc - prim) * c.dz * c.al;
-      double r = USE_RU ? c.ro * rzo : 0.0;
-      double rhs = (c.ga * h + r + c.alre * dxxvz - dzp) * c.dt;
-      double vin = rhs * cfv(SC_YSC) + ((cfv(SC_YDL) * vzm + vzc) + cfv(SC_YDU) * vzp);
-      sth<H32>(ruz, C + sb, iny ? h : 0.0);
-      stf(bz, C + sb, vin);
-    }
-    // ---- temperature (ExplicitTermsTemp.F90:52-93, ImplicitAndUpdateTemp.F90:41-54)
-    {
-      double htz = ((vzipm + vzip) * (tip + tc) - (vzm + vzc) * (tc + tim)) * c.udz4;
-      double hty = ((vyjp + vyjpm) * (tjp + tc) - (vyc + vym) * (tc + tjm)) * c.udy4;
-      double htx = ((vxp + vxc) * (tp + tc) - (vxc + vxm) * (tc + tm)) * udx3c * 0.25;
-      double dzzt = (tip - 2.0 * tc + tim) * c.udzq_pe;
-      double dyyt = (tjp - 2.0 * tc + tjm) * c.udyq_pe;
-      double h = -(htx + hty + htz) + dyyt + dzzt;
-      double dxxt = tp * apc + tc * acc + tm * amc;
-      double r = USE_RU ? c.ro * rto : 0.0;
-      double rhs = (c.ga * h + r + c.alpec * dxxt) * c.dt;
-      double vin = rhs * cfv(SC_TSC) + ((cfv(SC_TDL) * tm + tc) + cfv(SC_TDU) * tp);
-      sth<H32>(rut, C + sb, inx ? h : 0.0);
-      stf(bt, C + sb, inx ? vin : tc);   // row 1 gets the vz increment added by the solve pass
-    }
-    // march: plane i becomes i-1, plane i+1 becomes i
     vxim = vxc; vximp = vxp; vyim = vyc; vzim = vzc; tim = tc; prim = prc;
     vyjpim = vyjp;
     cur = nxt;
@@ -1152,14 +1195,15 @@ hipError_t launch_fused_update(const FusedArgs &a, hipStream_t st) {
 hipError_t launch_stencil_rhs(const FusedArgs &a, hipStream_t st) {
   const int64_t n = a.g.ncol * a.g.ldx;
   const int zplanes = a.zm > 0 ? a.zm : ZM_DEFAULT;
-  const int64_t ntile = (int64_t)((a.g.nzm + zplanes - 1) / zplanes) * a.g.nym * ((a.g.ldx + STENCIL_WG - 1) / STENCIL_WG);
+  const int64_t ntile = (int64_t)((a.g.nzm + zplanes - 1) / zplanes) * ((a.g.nym + SY_COLS - 1) / SY_COLS) *
+                        ((a.g.ldx + 63) / 64);
   const dim3 grid((unsigned)(((ntile + 7) / 8) * 8));
   const size_t shm = stencil_lds(a.g);
   const bool small = (uint64_t)n * sizeof(double) < (1ull << 32);
   const bool ru = a.sc.use_ru != 0, dev = a.ds != nullptr, h32 = a.h32 != 0;
 #define SR1(O, R, D, H)                                                                                  \
   if (std::is_same<O, uint32_t>::value == small && R == ru && D == dev && H == h32) {                  \
-    hipLaunchKernelGGL((k_stencil_rhs<O, R, D, H>), grid, dim3(STENCIL_WG), shm, st, a, a.f.vx, a.f.vy,  \
+    hipLaunchKernelGGL((k_stencil_rhs<O, R, D, H>), grid, dim3(SY_WG), shm, st, a, a.f.vx, a.f.vy,       \
                        a.f.vz, a.f.temp, a.f.pr, a.rux, a.ruy, a.ruz, a.rut, a.bx, a.by, a.bz, a.bt, a.stale, \
                        a.kc, a.fac);                                                                   \
     return hipGetLastError();                                                                          \
@@ -1223,7 +1267,7 @@ hipError_t launch_correct(const CorrectArgs &a, int which, hipStream_t st) {
   return hipGetLastError();
 }
 
-// LDS per block: WG*L doubles (fused update), SC_N*STENCIL_WG (stencil pass),
+// LDS per block: WG*L doubles (fused update), SC_N*64 + 2*SV_N*SY_WG (stencil pass),
 // SYS_N*RW_N*16*L (column solve); L <= 80 keeps all within gfx950's 160 KiB.
 int max_supported_L() { return 80; }
 size_t column_lds_need(const Geom &g) { return std::max(stencil_lds(g), solve_lds(g)); }
