@@ -1091,6 +1091,9 @@ static FusedArgs update_args(afid_ctx *c, int ns, bool dev) {
   a.ns = ns;
   a.h32 = c->mixed;
   a.zm = c->zm;
+  // The histories written here are read by the next substep only when its
+  // ro is nonzero: never after RK3 substep 3 (ro(1) = 0, TimeMarcher.F90:22-30).
+  a.keep_ru = c->rom[ns % c->p.nsst + 1] != 0.0;
   return a;
 }
 

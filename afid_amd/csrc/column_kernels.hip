@@ -558,7 +558,8 @@ __global__ void __launch_bounds__(WG)
 // group's block on the same XCD at the same time.  A plane's loads issue in
 // one batch before any value is used; the history rows outside the reference
 // loops (k = 1 and nx for vx/temp, nx for vy/vz), which nothing reads, are
-// kept at 0.
+// kept at 0.  Histories that no later substep reads (keep_ru = 0: RK3
+// substep 3, since the next step's substep 1 has ro = 0) are not stored.
 // k_column_solve: a wave owns four columns (16 chunk lanes each) and solves the
 // four systems in registers: A q_new = b', with the factor rows staged in LDS.
 // The vz increment at k=1, q_new - q_old, becomes the new stale row and the
@@ -627,6 +628,7 @@ __global__ void __launch_bounds__(SY_WG)
   const int gr = (int)(rest % ngr), ib = (int)(rest / ngr);
   if (DEV && a.ds->halted) return;
   const StepScalars c = DEV ? a.ds->sc[a.ns - 1] : a.sc;
+  const bool keep_ru = a.keep_ru != 0;
   for (int idx = threadIdx.x; idx < SC_N * 64; idx += SY_WG) {
     const int r = idx / 64, sl = min(q * 64 + idx - r * 64, ldx - 1);
     const int src = r < SC_XSC ? kc_row_of(r) * ldx : (((r - SC_XSC) / 3) * FR_N + fac_row_of((r - SC_XSC) % 3)) * ldx;
@@ -725,7 +727,7 @@ __global__ void __launch_bounds__(SY_WG)
         double rhs = (c.ga * q + r + c.alre * dxxvx - dxp) * c.dt;
         double vin = rhs * cfv(SC_XSC) + ((cfv(SC_XDL) * vxm + vxc) + cfv(SC_XDU) * vxp);
         double vbd = stale0 + vxc;
-        sth<H32>(rux, C + sb, inx ? q : 0.0);
+        if (keep_ru) sth<H32>(rux, C + sb, inx ? q : 0.0);
         stf(bx, C + sb, inx ? vin : vbd);
       }
       // ---- vy (ExplicitTermsVY.F90:52-98, ImplicitAndUpdateVY.F90:52-68); kmv/kpv clamps
@@ -743,7 +745,7 @@ __global__ void __launch_bounds__(SY_WG)
         double r = USE_RU ? c.ro * ryo : 0.0;
         double rhs = (c.ga * h + r + c.alre * dxxvy - dyp) * c.dt;
         double vin = rhs * cfv(SC_YSC) + ((cfv(SC_YDL) * vym + vyc) + cfv(SC_YDU) * vyp);
-        sth<H32>(ruy, C + sb, iny ? h : 0.0);
+        if (keep_ru) sth<H32>(ruy, C + sb, iny ? h : 0.0);
         stf(by, C + sb, vin);   // row nx-1 is outside the nxm system and never read
       }
       // ---- vz (ExplicitTermsVZ.F90:56-101, ImplicitAndUpdateVZ.F90:48-64)
@@ -761,7 +763,7 @@ __global__ void __launch_bounds__(SY_WG)
         double r = USE_RU ? c.ro * rzo : 0.0;
         double rhs = (c.ga * h + r + c.alre * dxxvz - dzp) * c.dt;
         double vin = rhs * cfv(SC_YSC) + ((cfv(SC_YDL) * vzm + vzc) + cfv(SC_YDU) * vzp);
-        sth<H32>(ruz, C + sb, iny ? h : 0.0);
+        if (keep_ru) sth<H32>(ruz, C + sb, iny ? h : 0.0);
         stf(bz, C + sb, vin);
       }
       // ---- temperature (ExplicitTermsTemp.F90:52-93, ImplicitAndUpdateTemp.F90:41-54)
@@ -776,7 +778,7 @@ __global__ void __launch_bounds__(SY_WG)
         double r = USE_RU ? c.ro * rto : 0.0;
         double rhs = (c.ga * h + r + c.alpec * dxxt) * c.dt;
         double vin = rhs * cfv(SC_TSC) + ((cfv(SC_TDL) * tm + tc) + cfv(SC_TDU) * tp);
-        sth<H32>(rut, C + sb, inx ? h : 0.0);
+        if (keep_ru) sth<H32>(rut, C + sb, inx ? h : 0.0);
         stf(bt, C + sb, inx ? vin : tc);   // row 1 gets the vz increment added by the solve pass
       }
     }

@@ -18,6 +18,7 @@ struct FusedArgs {
   int ns;
   int h32;                           // histories stored in FP32 (mixed precision)
   int zm;                            // z planes per stencil block (0: the default)
+  int keep_ru;                       // 0: no later substep reads this substep's histories; not stored
 };
 
 struct RoutineArgs {

@@ -33,8 +33,10 @@ HBM_PEAK_GBS = 8000.0
 # divergence 4; pressure solve 10 (5 passes); CorrectVelocity 7; CorrectPressure 3.
 WORDS = {
     # RK3: the first substep (ro = 0) does not read the four histories (13
-    # words), the other two do (17); the HIP-event average covers all three.
-    "stencil_rhs": (13 + 17 + 17) / 3,
+    # words); the third does not store them, since the next step's first
+    # substep does not read them (13); the second does both (17).  The
+    # HIP-event average covers all three.
+    "stencil_rhs": (13 + 17 + 13) / 3,
     "column_solve": 4 + 4,
     "fused_update": 9 + 4 + 4,
     "div_yfft_r2c": 3 + 1,

@@ -140,7 +140,11 @@ int afid_pressure_spectrum(afid_ctx *ctx, int stage, double *out);
 /* ---- fused time advance (TimeMarcher.F90:13-78) -----------------------------
  * One substep / one full TimeMarcher call with fused kernels: explicit terms,
  * implicit solves and updates in one pass; divergence + y-FFT; z-FFT; x
- * tridiagonals; inverse FFTs; corrections.  qcap/dq/hro/rhs are not kept. */
+ * tridiagonals; inverse FFTs; corrections.  qcap/dq/hro/rhs are not kept.
+ * With nsst = 3, substep 3 does not store its explicit terms in rux..rutemp
+ * (no later substep reads them: the next step's substep 1 has ro = 0), so
+ * after a full RK3 step those arrays hold substep 2's; the per-routine
+ * ImplicitAndUpdate* entries always store them. */
 int afid_substep(afid_ctx *ctx, int ns, double dt);
 int afid_time_marcher(afid_ctx *ctx, double dt);
 
