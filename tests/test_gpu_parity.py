@@ -342,6 +342,9 @@ EDGE_CASES = [
     dict(nxm=1279, nym=4, nzm=4, istr3=6, str3=6.0, ylen=1.0, zlen=1.0, ray=1e6, pra=1.0),
     dict(nxm=12, nym=768, nzm=6, istr3=6, str3=6.0, ylen=1.0, zlen=1.0, ray=1e6, pra=1.0),
     dict(nxm=10, nym=6, nzm=750, istr3=4, str3=3.0, ylen=1.0, zlen=1.0, ray=1e6, pra=1.0),
+    # fewer y columns than the stencil pass groups per block (4), and a ragged last group
+    dict(nxm=40, nym=3, nzm=8, istr3=6, str3=6.0, ylen=1.0, zlen=1.0, ray=1e6, pra=1.0),
+    dict(nxm=24, nym=5, nzm=9, istr3=4, str3=3.0, ylen=1.0, zlen=1.0, ray=1e6, pra=1.0),
 ]
 
 
