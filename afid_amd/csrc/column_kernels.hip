@@ -1066,54 +1066,86 @@ __global__ void k_divergence(RoutineArgs a) {
 }
 
 // CorrectVelocity.F90:28-45 and CorrectPressure.F90:32-52 from dph (= dphhalo).
-// which: 1 velocity, 2 pressure, 3 both (the fused path).  A block walks the
-// STRIP_W columns of one (strip, z) row (XCD-remapped, z fastest), so the y/z
-// neighbour columns of dph come from the L2 of the same XCD; the four per-k
-// coefficient rows are staged in LDS once per block.
-constexpr int CORRECT_MAX_WG = 512;
-__global__ void __launch_bounds__(CORRECT_MAX_WG) k_correct(CorrectArgs a, int which) {
-  extern __shared__ double cr[];   // udx3c, amphk, acphk, apphk rows
+// which: 1 velocity, 2 pressure, 3 both (the fused path).  The block shape is
+// that of k_stencil_rhs: SY_COLS waves on neighbouring y columns at the same
+// 64 stored positions, marching through zplanes z planes with dph of planes
+// i-1, i, i+1 in registers; each wave publishes its plane-i dph in LDS for its
+// y neighbours (double-buffered, one barrier per plane), so only the group's
+// edge columns load a y neighbour.  Positions past k = nxm are not written.
+constexpr int CZ_ROWS = 4;   // udx3c, amphk, acphk, apphk
+__global__ void __launch_bounds__(SY_WG) k_correct(CorrectArgs a, int which, int zplanes) {
+  extern __shared__ double sh[];
+  double *cr = sh;                    // [CZ_ROWS][64]
+  double *xv = sh + CZ_ROWS * 64;     // [2][SY_WG] published dph of plane i
   const Geom g = a.g;
-  const int nstrip = (g.nym + STRIP_W - 1) / STRIP_W;
-  const int64_t row = xcd_block((int64_t)nstrip * g.nzm);
-  if (row < 0 || (a.ds && a.ds->halted)) return;
-  const int strip = (int)(row / g.nzm), i = (int)(row - (int64_t)strip * g.nzm);
-  const int ldx = g.ldx;
-  for (int idx = threadIdx.x; idx < 4 * ldx; idx += blockDim.x) {
-    const int r = idx / ldx, s = idx - r * ldx;
+  const int L = g.L, ldx = g.ldx, nxm = g.nxm;
+  const int nq = (ldx + 63) / 64;
+  const int ngr = (g.nym + SY_COLS - 1) / SY_COLS;
+  const int nib = (g.nzm + zplanes - 1) / zplanes;
+  const int64_t tile = xcd_block((int64_t)nib * ngr * nq);
+  if (tile < 0 || (a.ds && a.ds->halted)) return;
+  const int q = (int)(tile % nq);
+  const int64_t rest = tile / nq;
+  const int gr = (int)(rest % ngr), ib = (int)(rest / ngr);
+  const StepScalars c = a.ds ? a.ds->sc[a.ns - 1] : a.sc;
+  for (int idx = threadIdx.x; idx < CZ_ROWS * 64; idx += SY_WG) {
+    const int r = idx / 64, sl = min(q * 64 + idx - r * 64, ldx - 1);
     const int src = r == 0 ? KC_UDX3C : (r == 1 ? KC_AMPHK : (r == 2 ? KC_ACPHK : KC_APPHK));
-    cr[idx] = a.kc[src * ldx + s];
+    cr[idx] = a.kc[src * ldx + sl];
   }
   __syncthreads();
-  const int im = wrapm(i - 1, g.nzm), ip = wrapp(i + 1, g.nzm);
+  const int lane = (int)threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  const int ncols = min(SY_COLS, g.nym - gr * SY_COLS);
+  const bool colok = w < ncols;
+  const int jc = colok ? gr * SY_COLS + w : g.nym - 1;
+  const int sraw = q * 64 + lane;
+  const int s = min(sraw, ldx - 1);
+  const int k = k_of_pos(s, L);
+  const bool live = colok && sraw < ldx && k <= nxm - 1;
+  const bool jp_in = w + 1 < ncols, jm_in = w > 0;
+  const int jm = wrapm(jc - 1, g.nym), jp = wrapp(jc + 1, g.nym);
+  const int skm = (k == 0 || k > nxm - 1) ? s : pos_of_k(k - 1, L);
+  const int skp = (k >= nxm - 1) ? s : pos_of_k(k + 1, L);
   const double *__restrict__ dph = a.dph;
-  const StepScalars c = a.ds ? a.ds->sc[a.ns - 1] : a.sc;
-  const int j0 = strip * STRIP_W, j1 = min(j0 + STRIP_W, g.nym);
-  for (int j = j0; j < j1; j++) {
-    const int jm = wrapm(j - 1, g.nym), jp = wrapp(j + 1, g.nym);
-    const int64_t C = col_off(j, i, g), CJM = col_off(jm, i, g), CJP = col_off(jp, i, g), CIM = col_off(j, im, g),
-                  CIP = col_off(j, ip, g);
-    for (int s = threadIdx.x; s < ldx; s += blockDim.x) {
-      const int k = k_of_pos(s, g.L);
-      if (k > g.nxm - 1) continue;
-      const int skm = k == 0 ? s : pos_of_k(k - 1, g.L);
-      const int skp = k == g.nxm - 1 ? s : pos_of_k(k + 1, g.L);
-      const double d = dph[C + s], dkm = dph[C + skm], djm = dph[CJM + s], dim_ = dph[CIM + s];
+  double *__restrict__ qx = a.vx, *__restrict__ qy = a.vy, *__restrict__ qz = a.vz, *__restrict__ qp = a.pr;
+  const double usukm = c.al * c.dt * cr[lane];
+  const double udy = c.al * c.dt * c.dy, udz = c.al * c.dt * c.dz;
+  const double be = c.al * c.beta;
+  const double amm = cr[64 + lane], acc = cr[128 + lane], app = cr[192 + lane];
+  const int i0 = ib * zplanes, i1 = min(i0 + zplanes, g.nzm);
+  double dim_ = dph[col_off(jc, wrapm(i0 - 1, g.nzm), g) + s];
+  double d = dph[col_off(jc, i0, g) + s];
+  const int tid = (int)threadIdx.x;
+  int par = 0;
+  for (int ic = i0; ic < i1; ic++) {
+    const int64_t C = col_off(jc, ic, g);
+    const double dip = dph[col_off(jc, wrapp(ic + 1, g.nzm), g) + s];
+    const double dkm = dph[C + skm], dkp = dph[C + skp];
+    double djm = 0.0, djp = 0.0;
+    if (!jm_in) djm = dph[col_off(jm, ic, g) + s];
+    if (!jp_in) djp = dph[col_off(jp, ic, g) + s];
+    double vxo = 0.0, vyo = 0.0, vzo = 0.0, pro = 0.0;
+    if (which & 1) { vxo = qx[C + s]; vyo = qy[C + s]; vzo = qz[C + s]; }
+    if (which & 2) pro = qp[C + s];
+    double *buf = xv + par * SY_WG;
+    buf[tid] = d;
+    __syncthreads();
+    if (jm_in) djm = buf[tid - 64];
+    if (jp_in) djp = buf[tid + 64];
+    par ^= 1;
+    if (live) {
       if (which & 1) {
-        double usukm = c.al * c.dt * cr[s];
-        double udy = c.al * c.dt * c.dy, udz = c.al * c.dt * c.dz;
-        a.vx[C + s] = a.vx[C + s] - (d - dkm) * usukm;
-        a.vy[C + s] = a.vy[C + s] - (d - djm) * udy;
-        a.vz[C + s] = a.vz[C + s] - (d - dim_) * udz;
+        qx[C + s] = vxo - (d - dkm) * usukm;
+        qy[C + s] = vyo - (d - djm) * udy;
+        qz[C + s] = vzo - (d - dim_) * udz;
       }
-      if (which & 2) {
-        double be = c.al * c.beta;
-        double amm = cr[ldx + s], acc = cr[2 * ldx + s], app = cr[3 * ldx + s];
-        a.pr[C + s] = a.pr[C + s] + d -
-                      be * ((dph[CIP + s] - 2.0 * d + dim_) * c.dzq + (dph[CJP + s] - 2.0 * d + djm) * c.dyq +
-                            (dph[C + skp] * app + d * acc + dkm * amm));
-      }
+      if (which & 2)
+        qp[C + s] = pro + d - be * ((dip - 2.0 * d + dim_) * c.dzq + (djp - 2.0 * d + djm) * c.dyq +
+                                    (dkp * app + d * acc + dkm * amm));
     }
+    dim_ = d;
+    d = dip;
   }
 }
 
@@ -1262,10 +1294,11 @@ hipError_t launch_divergence(const RoutineArgs &a, hipStream_t st) {
   return hipGetLastError();
 }
 hipError_t launch_correct(const CorrectArgs &a, int which, hipStream_t st) {
-  const int64_t nrow = (int64_t)((a.g.nym + STRIP_W - 1) / STRIP_W) * a.g.nzm;
-  const dim3 block((unsigned)std::min(CORRECT_MAX_WG, (a.g.ldx + 63) / 64 * 64));
-  hipLaunchKernelGGL(k_correct, dim3((unsigned)(((nrow + 7) / 8) * 8)), block, sizeof(double) * 4 * a.g.ldx, st, a,
-                     which);
+  const int zplanes = ZM_DEFAULT;
+  const int64_t nt = (int64_t)((a.g.nzm + zplanes - 1) / zplanes) * ((a.g.nym + SY_COLS - 1) / SY_COLS) *
+                     ((a.g.ldx + 63) / 64);
+  hipLaunchKernelGGL(k_correct, dim3((unsigned)(((nt + 7) / 8) * 8)), dim3(SY_WG),
+                     sizeof(double) * (CZ_ROWS * 64 + 2 * SY_WG), st, a, which, zplanes);
   return hipGetLastError();
 }
 
