@@ -369,7 +369,7 @@ __global__ void __launch_bounds__(WG)
       double r = USE_RU ? c.ro * ru : 0.0;
       double rhs = (c.ga * q + r + c.alre * dxxvx - dxp) * c.dt;
       const bool interior = k >= 1 && k <= nx - 2;
-      if (interior) stf(rux, n.C + sb, q);
+      if (interior && a.keep_ru) stf(rux, n.C + sb, q);
       double vin = rhs * scl + ((dl * vxm + vxc) + du * vxp);
       double vbd = (k == 0 ? stale : 0.0) + vxc;  // identity rows: rhs(1) stale, rhs(nx) zero
       bl[t * WG] = interior ? vin : vbd;
@@ -420,7 +420,7 @@ __global__ void __launch_bounds__(WG)
       double dyp = (prc - prjm) * c.udy_al;
       double r = USE_RU ? c.ro * ru : 0.0;
       double rhs = (c.ga * h + r + c.alre * dxxvy - dyp) * c.dt;
-      stf(ruy, n.C + sb, h);
+      if (a.keep_ru) stf(ruy, n.C + sb, h);
       bl[t * WG] = rhs * scl + ((dl * vym + vyc) + du * vyp);
       vym = vyc; vyc = vynext; vxk = vxkp; vxjmk = vxjmkp;
     }
@@ -471,7 +471,7 @@ __global__ void __launch_bounds__(WG)
       double dzp = (prc - prim) * c.dz * c.al;
       double r = USE_RU ? c.ro * ru : 0.0;
       double rhs = (c.ga * h + r + c.alre * dxxvz - dzp) * c.dt;
-      stf(ruz, n.C + sb, h);
+      if (a.keep_ru) stf(ruz, n.C + sb, h);
       bl[t * WG] = rhs * scl + ((dl * vzm + vzc) + du * vzp);
       vzm = vzc; vzc = vznext; vxk = vxkp; vximk = vximkp;
     }
@@ -519,7 +519,7 @@ __global__ void __launch_bounds__(WG)
       double r = USE_RU ? c.ro * ru : 0.0;
       double rhs = (c.ga * h + r + c.alpec * dxxt) * c.dt;
       const bool interior = k >= 1 && k <= nx - 2;
-      if (interior) stf(rut, n.C + sb, h);
+      if (interior && a.keep_ru) stf(rut, n.C + sb, h);
       double vin = rhs * scl + ((dl * tm + tc) + du * tp);
       double vbd = (k == 0 ? dvz0 : 0.0) + tc;
       bl[t * WG] = interior ? vin : vbd;

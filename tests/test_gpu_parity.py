@@ -187,12 +187,14 @@ def test_ab2_scheme(afid):
     cell.close()
 
 
+@pytest.mark.parametrize("path", ["split", "fused"])
 @pytest.mark.parametrize("nsst", [3, 1])
-def test_histories_kept_where_read(afid, nsst):
+def test_histories_kept_where_read(afid, nsst, path, monkeypatch):
     """Each substep stores its explicit terms in rux..rutemp as the reference
-    does, except RK3 substep 3: the next step's substep 1 has ro = 0 and never
+    does (both update paths), except RK3 substep 3: the next step's substep 1 has ro = 0 and never
     reads them, so that pass leaves substep 2's terms in place.  The step after
     it still matches the oracle, which stores them."""
+    monkeypatch.setenv("AFID_UPDATE", path)
     c = dict(SMALL_CASES[0])
     o, cell = make_pair(afid, c, nsst=nsst)
     o.init_linear_noise(4, 1e-3)
