@@ -65,6 +65,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 // flight before the first value is used) so that a thread never waits for one
 // load at a time.
 constexpr int LB = 8;
+constexpr int ZLB = 12;  // the z transform's batch: 12 loads cover a 4 x 768 tile per 256 threads in one
 constexpr int TB = 4;   // butterfly iterations per twiddle batch
 
 // In-place DIT stages over `nlines` (1 or 2) lines of length pl.n stored
@@ -212,7 +213,7 @@ __global__ void __launch_bounds__(256) k_yfft_forward(PoissonArgs a) {
     const int skp = live ? pos_of_k(k + 1, g.L) : s;
     const double u3 = a.udx3m[s];
     const int ip = wrapp(i + 1, g.nzm);
-    constexpr int YB = 4;   // rows per batch: 6 field loads each, all issued before any is used
+    constexpr int YB = 8;   // rows per batch: 6 field loads each, all issued before any is used
     for (int jb = jr; jb < N; jb += 32 * YB) {
       double z0[YB], z1[YB], y0[YB], y1[YB], x0[YB], x1[YB];
       int rv[YB];
@@ -299,18 +300,18 @@ __global__ void __launch_bounds__(256) k_zfft(PoissonArgs a, R sign) {
   const bool zsw = (N & 7) == 0;
   C *S = reinterpret_cast<C *>(a.S) + (int64_t)k * a.g.nzm * a.ldm;
   const int tot = TM * N;
-  for (int base = threadIdx.x; base < tot; base += 256 * LB) {
-    C v[LB];
-    int rv[LB];
+  for (int base = threadIdx.x; base < tot; base += 256 * ZLB) {
+    C v[ZLB];
+    int rv[ZLB];
 #pragma unroll
-    for (int b = 0; b < LB; b++) {   // all loads of the batch first
+    for (int b = 0; b < ZLB; b++) {   // all loads of the batch first
       const int ic = min(base + 256 * b, tot - 1);
       const int q = ic % TM, nn = ic / TM, m = m0 + q;
       v[b] = S[(int64_t)nn * a.ldm + min(m, a.nymh - 1)];
       rv[b] = a.pz.rev[nn];
     }
 #pragma unroll
-    for (int b = 0; b < LB; b++) {
+    for (int b = 0; b < ZLB; b++) {
       const int idx = base + 256 * b;
       if (idx >= tot) continue;
       const int q = idx % TM;
