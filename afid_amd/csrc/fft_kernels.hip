@@ -65,6 +65,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 // flight before the first value is used) so that a thread never waits for one
 // load at a time.
 constexpr int LB = 8;
+constexpr int YILB = 6;  // the y-inverse batch: two equal batches of its 12 loads per thread
 constexpr int ZLB = 12;  // the z transform's batch: 12 loads cover a 4 x 768 tile per 256 threads in one
 constexpr int TB = 4;   // butterfly iterations per twiddle batch
 
@@ -417,11 +418,11 @@ __global__ void __launch_bounds__(256) k_yfft_inverse(PoissonArgs a) {
   const int s0 = (int)(tile % ntile) * TK, i = (int)(tile / ntile);
   const bool hsw = (H & 7) == 0;
   const int nymh = a.nymh;
-  for (int base = threadIdx.x; base < TK * H; base += 256 * LB) {
-    C xa[LB], xb[LB], pw[LB];
-    int rv[LB];
+  for (int base = threadIdx.x; base < TK * H; base += 256 * YILB) {
+    C xa[YILB], xb[YILB], pw[YILB];
+    int rv[YILB];
 #pragma unroll
-    for (int b = 0; b < LB; b++) {   // all loads of the batch first
+    for (int b = 0; b < YILB; b++) {   // all loads of the batch first
       const int ic = min(base + 256 * b, TK * H - 1);
       const int q = ic / H, t = ic - q * H;
       const int k = min(k_of_pos(s0 + q, g.L), g.nxm - 1);
@@ -438,7 +439,7 @@ __global__ void __launch_bounds__(256) k_yfft_inverse(PoissonArgs a) {
       rv[b] = py.c.rev[t];
     }
 #pragma unroll
-    for (int b = 0; b < LB; b++) {
+    for (int b = 0; b < YILB; b++) {
       const int idx = base + 256 * b;
       if (idx >= TK * H) continue;
       const int q = idx / H, t = idx - q * H;
