@@ -2,8 +2,8 @@
 istr3 = 6, str3 = 6), ylen = zlen = 1, Ra = 1e8, Pr = 1, RK3, seeded conduction
 profile plus +-1e-3 noise.  Three main.F90 steps with the CFL-limited dt rule
 (the first two at dtmin, as main.F90 starts; the third at the dt the CFL number
-allows), the GPU path against the oracle on the same input: the same error
-code and dt per step, per-field rel-L2 <= 1e-11 (pr modulo its mean), and the
+allows), the GPU path (host-driven steps, and steps from captured graphs)
+against the oracle on the same input: the same error code and dt per step, per-field rel-L2 <= 1e-11 (pr modulo its mean), and the
 maximum local divergence below 1e-10 in both.  The oracle holds about 40 GB of
 host memory and runs the three steps in under a minute on 16 threads.
 """
@@ -46,6 +46,22 @@ def test_production_size_steps_match_oracle():
         assert e < 1e-11
         assert cell.CheckDivergence() < 1e-10
         assert o.divmax() < 1e-10
+        # the same three steps from captured step graphs (afid_steps), the timed path of bench.py
+        dev = Cell(CellParams(nsst=3, alx3=1.0, **CELL))
+        try:
+            dev.init_linear_noise(2, 1e-3)
+            err, dt, done = dev.steps(0, 3, *CTL)
+            assert (err, done) == (0, 3)
+            assert dt == pytest.approx(dtg, rel=1e-12)
+            for nm in ("vx", "vy", "vz", "temp"):
+                e = rel_l2(dev.download(nm), o.interior(nm))
+                print(f"graph steps {nm}: rel-L2 {e:.2e}", flush=True)
+                assert e < 1e-11, nm
+                # dt and the factorisations come from the device here, from the host
+                # there: the two GPU paths agree to rounding, not bit for bit
+                assert rel_l2(dev.download(nm), cell.download(nm)) < 1e-12, nm
+        finally:
+            dev.close()
     finally:
         cell.close()
         o.close()
