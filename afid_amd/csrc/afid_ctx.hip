@@ -655,7 +655,6 @@ int afid_create(afid_ctx **out, const afid_params *p, const afid_metrics *m) {
   c->nymh = g.nym / 2 + 1;
   c->ldm = (c->nymh + 3) / 4 * 4;
   create_grid(c);
-  c->singular_mode = zero_mode_info(c);
   if (m) {  // caller-supplied metrics override the computed ones (1-based arrays)
     const int nx = g.nx, nxm = g.nxm;
     auto take = [](std::vector<double> &v, const double *src, int n) {
@@ -670,6 +669,8 @@ int afid_create(afid_ctx **out, const afid_params *p, const afid_metrics *m) {
     take(c->amphk, m->amphk, nxm); take(c->acphk, m->acphk, nxm); take(c->apphk, m->apphk, nxm);
     take(c->ak1, m->ak1, g.nzm); take(c->ak2, m->ak2, g.nym);
   }
+  // decided on the coefficients the solve actually uses (caller metrics included)
+  c->singular_mode = zero_mode_info(c);
   int rc = AFID_OK;
 #define TRY(x)                  \
   do {                          \
@@ -1141,7 +1142,17 @@ static int refine_projection(afid_ctx *c, int ns, bool dev) {
 }
 
 int afid_time_marcher(afid_ctx *c, double dt) {
-  if (!c) return AFID_ERR_ARG;
+  if (!c || dt <= 0) return AFID_ERR_ARG;
+  // every substep's factorisation is checked before the first one runs, so a
+  // refused step (AFID_NUM_PIVOT) leaves the fields as they were, as the
+  // device step path does
+  {
+    std::vector<double> rows(fac_bytes(c) / sizeof(double));
+    for (int q = 1; q <= c->p.nsst && q <= 3; q++) {
+      int rc = build_factors(c, c->alm[q], dt, rows.data());
+      if (rc) return rc;
+    }
+  }
   int ns = 1;
   for (; ns <= c->p.nsst && ns <= 3; ns++) {
     int rc = afid_substep(c, ns, dt);

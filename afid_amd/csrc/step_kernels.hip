@@ -6,6 +6,8 @@
 //   k_factor      the wall-normal systems of each substep for that dt
 //                 (SolveImpEqnUpdate_X.F90:27-44, _YZ.F90:25-38, _Temp.F90:31-47)
 //   k_step_end    time += dt, ntime += 1, halt after an error code
+// A factorisation that would need a row interchange halts the cell in
+// k_factor, before the substeps of that step run.
 #include "afid_internal.h"
 #include "kernels.h"
 
@@ -106,7 +108,12 @@ __global__ void __launch_bounds__(256) k_factor(FactorArgs a) {
       l[0] = 0.0;
       u[0] = 1.0;
       for (int k = 1; k < n; k++) {
-        if (fabs(u[k - 1]) < fabs(dl[k])) a.ds->pivot = 1;
+        // dgttrf would interchange rows here: the pivot-free LU is refused, and
+        // the cell halts before any substep of this step changes a field
+        if (fabs(u[k - 1]) < fabs(dl[k])) {
+          a.ds->pivot = 1;
+          a.ds->halted = 1;
+        }
         l[k] = dl[k] / u[k - 1];
         u[k] = 1.0 - l[k] * du[k - 1];
       }

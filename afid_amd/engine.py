@@ -19,6 +19,7 @@ from __future__ import annotations
 import ctypes
 import os
 from dataclasses import dataclass, field
+from typing import Optional
 
 import numpy as np
 
@@ -51,6 +52,17 @@ class _Params(ctypes.Structure):
 
 PREC_FP64, PREC_MIXED = 0, 1
 
+# afid_metrics members in declaration order: 1-D arrays with their lengths
+# ("nx", "nxm", "nzm" or "nym"); an omitted one keeps the library's own grid.
+METRIC_NAMES = (("xc", "nx"), ("xm", "nx"), ("g3rc", "nx"), ("g3rm", "nx"), ("udx3c", "nx"), ("udx3m", "nx"),
+                ("ap3ck", "nx"), ("ac3ck", "nx"), ("am3ck", "nx"), ("ap3sk", "nx"), ("ac3sk", "nx"),
+                ("am3sk", "nx"), ("ap3ssk", "nx"), ("ac3ssk", "nx"), ("am3ssk", "nx"),
+                ("amphk", "nxm"), ("acphk", "nxm"), ("apphk", "nxm"), ("ak1", "nzm"), ("ak2", "nym"))
+
+
+class _Metrics(ctypes.Structure):
+    _fields_ = [(nm, ctypes.POINTER(ctypes.c_double)) for nm, _ in METRIC_NAMES]
+
 
 _lib = None
 
@@ -66,7 +78,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     vp, i, d, i64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int64
     pd = ctypes.POINTER(d)
     sig = {
-        "afid_create": ([ctypes.POINTER(vp), ctypes.POINTER(_Params), vp], i),
+        "afid_create": ([ctypes.POINTER(vp), ctypes.POINTER(_Params), ctypes.POINTER(_Metrics)], i),
         "afid_destroy": ([vp], i),
         "afid_strerror": ([i], ctypes.c_char_p),
         "afid_last_hip_error": ([vp], i),
@@ -247,12 +259,31 @@ def batch_steps(cells, nsteps: int):
 class Cell:
     """One Rayleigh-Benard cell resident on the GPU."""
 
-    def __init__(self, params: CellParams):
+    def __init__(self, params: CellParams, metrics: Optional[dict] = None):
+        """metrics: caller-supplied 1-D grid arrays by param.F90 name (1-based
+        arrays given from index 1, as afid_metrics takes them); they replace the
+        library's own CreateGrid / InitPressureSolver values."""
         self.L = load_library()
         self.params = params
         self.h = ctypes.c_void_p()
         cp = params.to_c()
-        self._check(self.L.afid_create(ctypes.byref(self.h), ctypes.byref(cp), None), "afid_create")
+        mp = None
+        if metrics:
+            sizes = {"nx": params.nxm + 1, "nxm": params.nxm, "nym": params.nym, "nzm": params.nzm}
+            unknown = set(metrics) - {nm for nm, _ in METRIC_NAMES}
+            if unknown:
+                raise ValueError(f"unknown metric arrays: {sorted(unknown)}")
+            self._metric_arrays = {}
+            m = _Metrics()
+            for nm, n in METRIC_NAMES:
+                if nm in metrics:
+                    a = np.ascontiguousarray(metrics[nm], dtype=np.float64)
+                    if a.shape != (sizes[n],):
+                        raise ValueError(f"metric {nm}: {sizes[n]} values expected, got {a.shape}")
+                    self._metric_arrays[nm] = a
+                    setattr(m, nm, _dp(a))
+            mp = ctypes.byref(m)
+        self._check(self.L.afid_create(ctypes.byref(self.h), ctypes.byref(cp), mp), "afid_create")
         dims = (ctypes.c_int64 * 6)()
         self._check(self.L.afid_get_dims(self.h, dims), "afid_get_dims")
         self.nx, self.nxm, self.nym, self.nzm, self.ldx, self.device_bytes = list(dims)

@@ -6,9 +6,10 @@ estimated device bytes of the admitted cells stay within the budget
 (afid_estimate_bytes), keeps every admitted cell resident, and keeps each one's
 stream fed with chunks of captured steps (afid_steps_async / afid_steps_more),
 so that work of cells admitted at different times overlaps on the GPU.  A cell
-finishes when all its steps have run (or its step control stopped with an
-error code, as main.F90 does); its memory then goes back to the budget and the
-next waiting request is admitted.  Each cell's result depends only on its own
+finishes when all its steps have run, or as soon as its step control has
+stopped with an error code, as main.F90 does; its memory then goes back to the
+budget and the next waiting request is admitted.  A library error raised for
+one cell (a refused factorisation, say) ends that request only.  Each cell's result depends only on its own
 request: the cells share no device data.
 
 The serving stream: Poisson arrivals at lam requests/s; nxm uniform in
@@ -159,24 +160,33 @@ class Server:
             self.peak_active = max(self.peak_active, len(active))
             for rid in list(active):
                 a = active[rid]
-                if a.cell.busy():
-                    continue
-                progressed = True
-                if a.queued < a.req.spec.steps:
-                    # later chunks are twice the first: fewer idle gaps between polls
-                    n = min(2 * self.chunk, a.req.spec.steps - a.queued)
-                    a.cell.steps_more(n)
-                    a.queued += n
-                    continue
-                err, _, ndone = a.cell.steps_wait()
-                div = a.cell.CheckDivergence()
-                if self.on_finish:
-                    self.on_finish(a.req, a.cell)
+                try:
+                    if a.cell.busy():
+                        continue
+                    progressed = True
+                    # the stream is idle: reading the step state costs one small copy
+                    err, _, ndone = a.cell.steps_wait()
+                    if err == 0 and a.queued < a.req.spec.steps:
+                        # later chunks are twice the first: fewer idle gaps between polls
+                        n = min(2 * self.chunk, a.req.spec.steps - a.queued)
+                        a.cell.steps_more(n)
+                        a.queued += n
+                        continue
+                    # all steps ran, or the step control halted with an error code
+                    # (main.F90 stops there): the request ends now
+                    div = a.cell.CheckDivergence()
+                    if self.on_finish:
+                        self.on_finish(a.req, a.cell)
+                    fin = Completion(rid, a.req.arrival, a.admitted, self.clock() - t0, ndone, err, div,
+                                     a.req.spec.params.cells * ndone)
+                except AfidError as e:
+                    # e.g. AFID_NUM_PIVOT from a step: this request fails, the others go on
+                    t = self.clock() - t0
+                    fin = Completion(rid, a.req.arrival, a.admitted, t, 0, e.code, 0.0, 0, str(e))
                 a.cell.close()
                 used -= a.nbytes
                 del active[rid]
-                done.append(Completion(rid, a.req.arrival, a.admitted, self.clock() - t0, ndone, err, div,
-                                       a.req.spec.params.cells * ndone))
+                done.append(fin)
             if not progressed:
                 if pending and not waiting and not active:
                     time.sleep(max(0.0, min(pending[0].arrival - now, 0.01)))

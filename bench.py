@@ -8,8 +8,9 @@ dt selection, three substeps.  The timed steps run on the device from one
 captured graph per step (afid_steps); inputs are resident in HBM before timing.
 
 Prints one JSON line (metric Mcell-updates/s = nxm*nym*nzm*steps/s/1e6) with a
-roofline block for the dominant kernel (HIP events on the engine's stream) and a
-CPU baseline from the oracle on a bounded sample of the same workload.
+roofline block for the dominant kernel (HIP events on the engine's stream), a
+CPU baseline from the reference's own routines on a bounded sample of the same
+workload, and beside it the C restatement on one full step of the whole cell.
 """
 from __future__ import annotations
 
@@ -83,22 +84,63 @@ CONFIGS = {
 }
 
 
-def cpu_baseline(cfg: dict, threads: int) -> dict:
-    """Oracle (C restatement, OpenMP) on one RK3 substep of the same cell."""
+REF_SAMPLE_NYZ = 128     # lateral size of the reference-routine sample (same x grid as the cell)
+REF_SAMPLE_STEPS = 2
+
+
+def cpu_baseline_reference(cfg: dict) -> dict | None:
+    """The reference's own routines (oracle/_ref/ref_harness, built from the
+    reference sources: TimeMarcher, ExplicitTerms*, ImplicitAndUpdate*,
+    SolveImpEqnUpdate_* with LAPACK, corrections, CalcMaxCFL and the main.F90
+    dt rule) on a bounded sample of the workload: the cell's x grid, Ra and Pr
+    with ny = nz = REF_SAMPLE_NYZ, REF_SAMPLE_STEPS main.F90 steps from the same
+    seeded state.  One MPI rank on one core: the harness stands in the C
+    restatement's Poisson solve for the FFTW one (FFTW3 is not in the image),
+    which needs the whole domain on one rank.  The step loop's own wall time is
+    used (written by the harness), not process start-up or file I/O."""
+    import tempfile
+    from oracle import oracle as O
+    if not O.ref_available():
+        return None
+    p = {k: v for k, v in cfg.items() if k != "seed"}
+    p.update(nym=REF_SAMPLE_NYZ, nzm=REF_SAMPLE_NYZ, nsst=3, alx3=1.0, inslws=1, inslwn=1)
+    with tempfile.TemporaryDirectory() as wd:
+        o = O.Oracle(**p)
+        o.init_linear_noise(cfg["seed"], 1e-3)
+        with open(os.path.join(wd, "fields_in.bin"), "wb") as f:
+            for n in ("vx", "vy", "vz", "temp", "pr", "rux", "ruy", "ruz", "rutemp", "rhs"):
+                f.write(o.arr(n).tobytes())
+        o.close()
+        O.run_ref(wd, p, mode=3, nsteps=REF_SAMPLE_STEPS, dt=1e-3, idtv=1, dtmin=1e-8, dtmax=1e-2,
+                  limitcfl=1.2, timeout=600)
+        with open(os.path.join(wd, "march_time.txt")) as f:
+            t = float(f.read().split()[0])
+    cells = p["nxm"] * p["nym"] * p["nzm"]
+    return {"value": round(cells * REF_SAMPLE_STEPS / t / 1e6, 3), "unit": "Mcell-updates/s", "cores": 1,
+            "kind": "reference",
+            "sample": f"the reference's routines (amdflang -O2, MKL LAPACK, 1 MPI rank) on "
+                      f"{p['nxm']}x{p['nym']}x{p['nzm']} (the cell's x grid, lateral size cut to "
+                      f"{REF_SAMPLE_NYZ}), {REF_SAMPLE_STEPS} main.F90 steps from the seeded state, step loop "
+                      f"{t:.1f} s; the Poisson solve is the C restatement's (FFTW3 absent)"}
+
+
+def cpu_baseline_port(cfg: dict, threads: int) -> dict:
+    """The C restatement (oracle, OpenMP over z planes) on one full main.F90 step
+    (CalcMaxCFL, dt, all three RK3 substeps) of the whole cell."""
     from oracle import oracle as O
     O.lib().afo_set_threads(threads)
     p = {k: v for k, v in cfg.items() if k != "seed"}
     o = O.Oracle(nsst=3, alx3=1.0, **p)
     o.init_linear_noise(cfg["seed"], 1e-3)
     t0 = time.perf_counter()
-    o.substep(1, 1e-4)
+    o.step(0, 1, 1e-3, 1e-8, 1e-2, 1.2)
     t = time.perf_counter() - t0
     cells = p["nxm"] * p["nym"] * p["nzm"]
     o.close()
-    return {"value": round(cells / (3 * t) / 1e6, 3), "unit": "Mcell-updates/s", "cores": threads,
-            "kind": "port",
-            "sample": f"one RK3 substep (of 3 per step) of the same {p['nxm']}x{p['nym']}x{p['nzm']} cell, "
-                      f"{t:.1f} s on {threads} OpenMP threads; rate scaled to full RK3 steps"}
+    return {"value": round(cells / t / 1e6, 3), "unit": "Mcell-updates/s", "cores": threads,
+            "kind": "port", "host_cpus": os.cpu_count(),
+            "sample": f"one full RK3 step (3 substeps) of the same {p['nxm']}x{p['nym']}x{p['nzm']} cell, "
+                      f"{t:.1f} s on {threads} OpenMP threads"}
 
 
 def copy_calibration(mib: int) -> None:
@@ -233,7 +275,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="cfg2", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
+    # the host's CPU share: OMP_NUM_THREADS where set (16 per GPU on the GPU hosts),
+    # else every CPU this process may run on
+    ap.add_argument("--cpu-threads", type=int,
+                    default=int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0)))
     ap.add_argument("--profile-steps", type=int, default=2)
     ap.add_argument("--workload", default="cell", choices=("cell", "sweep", "serve", "mixed"),
                     help="cell: one production cell (default); sweep: 64-cell batched parameter sweep; "
@@ -300,9 +345,15 @@ def main():
     substep_alg = SUBSTEP_WORDS * 8 * cells / (substep_ms * 1e-3) / 1e9
     cell.close()
 
-    cpu = None
+    cpu = cpu_port = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(cfg, args.cpu_threads)
+        print("cpu baseline: reference routines on a bounded sample", file=sys.stderr, flush=True)
+        cpu = cpu_baseline_reference(cfg)
+        print(f"cpu baseline: C restatement on {args.cpu_threads} threads, one full step", file=sys.stderr,
+              flush=True)
+        cpu_port = cpu_baseline_port(cfg, args.cpu_threads)
+        if cpu is None:       # reference harness not built: the port is the only baseline
+            cpu, cpu_port = cpu_port, None
 
     out = {
         "metric": "Mcell-updates/s", "value": round(value, 2), "unit": "Mcell-updates/s", "n_gpus": 1,
@@ -313,6 +364,7 @@ def main():
                                f"str3={p['str3']} Ra={p['ray']:g} Pr={p['pra']:g} RK3 CFL dt", "cells": cells},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "cpu_baseline_port": cpu_port,
         "substep": {"ms": round(substep_ms, 3), "alg_GBps_65words": round(substep_alg, 1),
                     "frac_hbm_65words": round(substep_alg / HBM_PEAK_GBS, 4)},
         "kernels_ms": {k: round(v["avg_ms"], 4) for k, v in kern.items()},

@@ -18,7 +18,8 @@
 !   inslws inslwn
 !   mode nsteps dt idtv dtmin dtmax limitcfl
 ! mode 1: dump grid.  mode 2: per-routine stages from fields_in.bin.
-! mode 3: march nsteps steps from fields_in.bin.
+! mode 3: march nsteps steps from fields_in.bin; the step loop's wall time
+!         goes to march_time.txt.
 ! mode 4: the reference interp of interp_old.bin (Fortran (nxo, nyo-1, nzo-1))
 !         onto this grid; interp.in holds nxo nyo nzo istro3 stro3 intvar.
 !         The old window is ReadFlowField's at one rank; the padded old array
@@ -89,6 +90,7 @@ program ref_harness
   use harness_glue
   implicit none
   integer :: mode, nsteps, u
+  integer(8) :: clk0, clk1, clkrate
   real(8) :: dtin, dtmin_, dtmax_, limitcfl_, instcfl, dmax
   real(8) :: nu_diag(2)
 
@@ -189,6 +191,7 @@ program ref_harness
   else if (mode == 3) then
     call read_fields()
     open(newunit=u, file='march_diag.bin', access='stream', form='unformatted', status='replace')
+    call system_clock(clk0, clkrate)
     do ntime = 0, nsteps - 1
       ! dt selection restated from main.F90:171-189
       call CalcMaxCFL(instcfl)
@@ -211,6 +214,11 @@ program ref_harness
       call CheckDivergence(dmax)
       write(u) dt, instcfl, dmax
     end do
+    call system_clock(clk1)
+    close(u)
+    ! wall time of the step loop alone (main.F90:168,226-237 times the same span)
+    open(newunit=u, file='march_time.txt', status='replace')
+    write(u, '(es24.16)') dble(clk1 - clk0) / dble(clkrate)
     close(u)
     open(newunit=u, file='march_out.bin', access='stream', form='unformatted', status='replace')
     write(u) vx, vy, vz, temp, pr

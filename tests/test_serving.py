@@ -48,9 +48,14 @@ def test_request_and_sweep_distributions():
 
 
 class FakeCell:
-    """Stand-in for engine.Cell: each step takes 0.2 ms of wall time."""
+    """Stand-in for engine.Cell: each step takes 0.2 ms of wall time.  A cell
+    whose nxm is in HALT stops with errorcode 166 after 5 steps (its later steps
+    are no-ops); one whose nxm is in FAIL raises AfidError from steps_more."""
     live = 0
     peak = 0
+    HALT = ()
+    FAIL = ()
+    more_calls = {}
 
     def __init__(self, params):
         self.params = params
@@ -73,6 +78,10 @@ class FakeCell:
         self._queue(n)
 
     def steps_more(self, n):
+        FakeCell.more_calls[self.params.nxm] = FakeCell.more_calls.get(self.params.nxm, 0) + 1
+        if self.params.nxm in FakeCell.FAIL:
+            from afid_amd.engine import AfidError
+            raise AfidError("steps_more: implicit factorisation needs a row interchange (code 1, hip 0)", 1)
         self._queue(n)
 
     def busy(self):
@@ -81,6 +90,8 @@ class FakeCell:
     def steps_wait(self):
         while self.busy():
             time.sleep(1e-4)
+        if self.params.nxm in FakeCell.HALT:
+            return 166, 1e-3, min(self.queued, 5)
         return 0, 1e-3, self.queued
 
     def CheckDivergence(self):
@@ -111,6 +122,28 @@ def test_server_admission_by_budget():
     assert srv.peak_bytes <= 100_000 and FakeCell.peak == srv.peak_active >= 2
     s = summarize(done)
     assert s["requests"] == 8 and s["p99_latency_s"] >= s["p50_latency_s"] > 0
+
+
+def test_server_ends_halted_and_failing_cells_early():
+    """A cell whose step control halted gets no further chunks and gives its
+    memory back at once; a library error of one cell ends that request only."""
+    FakeCell.live = FakeCell.peak = 0
+    FakeCell.HALT, FakeCell.FAIL, FakeCell.more_calls = (41,), (42,), {}
+    try:
+        reqs = [_req(0, 0.0, 41, 400), _req(1, 0.0, 42, 400), _req(2, 0.0, 40, 60), _req(3, 0.0, 43, 60)]
+        srv = Server(budget_bytes=10_000, chunk=8, make_cell=FakeCell, estimate=lambda p: 1000)
+        done = {c.rid: c for c in srv.run(reqs)}
+        assert sorted(done) == [0, 1, 2, 3] and FakeCell.live == 0
+        halted, failed = done[0], done[1]
+        assert halted.errorcode == 166 and halted.steps_done == 5 and not halted.refused
+        assert FakeCell.more_calls.get(41, 0) == 0          # halted after its first chunk
+        assert failed.refused and failed.errorcode == 1 and failed.steps_done == 0
+        assert done[2].steps_done == 60 and done[3].steps_done == 60
+        assert done[2].errorcode == 0 == done[3].errorcode
+        s = summarize(list(done.values()))
+        assert s["refused"] == 1 and 166 in s["errorcodes"]
+    finally:
+        FakeCell.HALT, FakeCell.FAIL = (), ()
 
 
 def test_server_rejects_request_above_budget():
