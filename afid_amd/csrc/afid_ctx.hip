@@ -46,10 +46,8 @@ struct afid_ctx {
   int singular_mode = 0;       // zgttrf info of the (0,0) pressure mode (>0: reference aborts)
   // device state
   double *f[AFID_NFIELDS] = {nullptr};
-  double *fn[4] = {nullptr};   // second buffers of vx, vy, vz, temp for the fused update
-  double *bp[4] = {nullptr};   // split update: right-hand sides b + A q_old of vx, vy, vz, temp
-  int split = 1;               // 1: stencil pass + column-solve pass (fields updated in place),
-                               // 0: one fused pass into the second buffers fn
+  double *bp[4] = {nullptr};   // substep update: right-hand sides b + A q_old of vx, vy, vz, temp;
+                               // after the column solve, u* (the velocities before the correction)
   // device step control (afid_steps): step state, factorisation inputs/scratch,
   // factor rows per substep, and the captured graph of one step
   DevStep *dstep = nullptr;
@@ -75,7 +73,7 @@ struct afid_ctx {
   double *pin_fac = nullptr;   // pinned staging for factor uploads
   hipEvent_t fac_ev = nullptr;
   double2 *S = nullptr;
-  double *cp = nullptr;
+  double *piv = nullptr;       // pressure x-solve pivots at chunk starts (per mode)
   double *nat = nullptr;       // natural-order arrays: g3rc, xc, amphk, acphk, apphk, ak1, ak2, dxc, dxm, lem, lec
   double *partial = nullptr, *out = nullptr;
   double *stage = nullptr;
@@ -505,7 +503,8 @@ int kerr(afid_ctx *c, hipError_t e) {
   return AFID_OK;
 }
 
-PoissonArgs poisson_args(afid_ctx *c, int from_velocity, int ns = 0, bool dev = false) {
+// from_ustar: the divergence is taken of u* in the b buffers (substep path)
+PoissonArgs poisson_args(afid_ctx *c, int from_velocity, int ns = 0, bool dev = false, bool from_ustar = false) {
   PoissonArgs a;
   a.ds = dev ? c->dstep : nullptr;
   a.ns = ns;
@@ -513,7 +512,7 @@ PoissonArgs poisson_args(afid_ctx *c, int from_velocity, int ns = 0, bool dev = 
   a.nymh = c->nymh;
   a.ldm = c->ldm;
   a.S = c->S;
-  a.cp = c->cp;
+  a.piv = c->piv;
   a.kc = c->kc;
   a.amphk = c->d_amphk;
   a.acphk = c->d_acphk;
@@ -521,6 +520,11 @@ PoissonArgs poisson_args(afid_ctx *c, int from_velocity, int ns = 0, bool dev = 
   a.ak1 = c->d_ak1;
   a.ak2 = c->d_ak2;
   a.f = fields(c);
+  if (from_ustar) {
+    a.f.vx = c->bp[0];
+    a.f.vy = c->bp[1];
+    a.f.vz = c->bp[2];
+  }
   a.dph = c->f[AFID_DPH];
   a.udx3m = c->kc + KC_UDX3M * c->g.ldx;
   a.sc = scalars(c);
@@ -531,9 +535,9 @@ PoissonArgs poisson_args(afid_ctx *c, int from_velocity, int ns = 0, bool dev = 
   return a;
 }
 
-int run_poisson(afid_ctx *c, int from_velocity, int ns = 0, bool dev = false) {
+int run_poisson(afid_ctx *c, int from_velocity, int ns = 0, bool dev = false, bool from_ustar = false) {
   if (c->singular_mode) return AFID_NUM_SINGULAR;
-  PoissonArgs a = poisson_args(c, from_velocity, ns, dev);
+  PoissonArgs a = poisson_args(c, from_velocity, ns, dev, from_ustar);
   int rc;
   {
     Timed t(c, from_velocity ? "div_yfft_r2c" : "yfft_r2c");
@@ -610,7 +614,6 @@ int afid_destroy(afid_ctx *c) {
     hipEventDestroy(pt.b);
   }
   for (int q = 0; q < AFID_NFIELDS; q++) hipFree(c->f[q]);
-  for (int q = 0; q < 4; q++) hipFree(c->fn[q]);
   for (int q = 0; q < 4; q++) hipFree(c->bp[q]);
   for (int q = 0; q < 3; q++) hipFree(c->fac_dev[q]);
   if (c->step_graph) hipGraphExecDestroy(c->step_graph);
@@ -622,7 +625,7 @@ int afid_destroy(afid_ctx *c) {
   hipFree(c->d_kslab);
   hipFree(c->slab_buf);
   for (int q = 0; q < 4; q++) hipFree(c->fac_sub[q]);
-  void *ps[] = {c->stale, c->kc, c->fac_x, c->S, c->cp, c->nat, c->partial, c->out, c->stage,
+  void *ps[] = {c->stale, c->kc, c->fac_x, c->S, c->piv, c->nat, c->partial, c->out, c->stage,
                 c->dpy.tw, c->dpy.rev, c->dpy.post, c->dpz.tw, c->dpz.rev, c->dpz.post};
   for (void *q : ps) hipFree(q);
   if (c->pin_fac) hipHostFree(c->pin_fac);
@@ -648,7 +651,7 @@ int afid_create(afid_ctx **out, const afid_params *p, const afid_metrics *m) {
   g.L = (g.nx + P - 1) / P;
   g.ldx = P * g.L;
   g.ncol = (int64_t)g.nym * g.nzm;
-  if (g.L > max_supported_L()) {
+  if (g.L > max_supported_L() || g.nxm > xsolve_max_rows()) {
     delete c;
     return AFID_ERR_ARG;
   }
@@ -709,21 +712,20 @@ int afid_create(afid_ctx **out, const afid_params *p, const afid_metrics *m) {
   }
   for (int q : {AFID_VX, AFID_VY, AFID_VZ, AFID_TEMP, AFID_PR, AFID_RUX, AFID_RUY, AFID_RUZ, AFID_RUTEMP, AFID_DPH})
     TRY(alloc_field(c, &c->f[q]));
-  if (const char *e = getenv("AFID_UPDATE")) c->split = strcmp(e, "fused") != 0;
   c->mixed = p->precision == AFID_PREC_MIXED;
-  if (c->mixed && !c->split) TRY(AFID_ERR_ARG);   // the mixed mode runs on the split update
-  if (c->split)
-    for (int q = 0; q < 4; q++) TRY(alloc_field(c, &c->bp[q]));
-  else
-    for (int q = 0; q < 4; q++) TRY(alloc_field(c, &c->fn[q]));
+  for (int q = 0; q < 4; q++) TRY(alloc_field(c, &c->bp[q]));
   TRYH(hipMalloc(&c->stale, sizeof(double) * g.ncol));
   c->bytes_held += sizeof(double) * g.ncol;
   TRYH(hipMemsetAsync(c->stale, 0, sizeof(double) * g.ncol, c->st));
   {
     size_t ns = (size_t)g.nxm * g.nzm * c->ldm;
     if (hipMalloc(&c->S, sizeof(double2) * ns) != hipSuccess) TRY(AFID_ERR_NOMEM);
-    if (hipMalloc(&c->cp, sizeof(double) * ns) != hipSuccess) TRY(AFID_ERR_NOMEM);
-    c->bytes_held += ns * (sizeof(double2) + sizeof(double));
+    c->bytes_held += ns * sizeof(double2);
+    int np, lm;
+    xsolve_shape(g.nxm, &np, &lm);
+    const size_t npiv = (size_t)c->nymh * g.nzm * np;
+    if (hipMalloc(&c->piv, sizeof(double) * npiv) != hipSuccess) TRY(AFID_ERR_NOMEM);
+    c->bytes_held += npiv * sizeof(double);
   }
   // interleaved coefficient rows
   {
@@ -786,6 +788,9 @@ int afid_create(afid_ctx **out, const afid_params *p, const afid_metrics *m) {
     c->d_lem = c->nat + off[9];
     c->d_lec = c->nat + off[10];
   }
+  // pressure x-solve pivots at the chunk starts: they depend on the grid and the
+  // wavenumbers only, not on dt
+  TRYH(launch_xsolve_pivots(poisson_args(c, 0), c->piv, c->st));
   TRYH(hipMalloc(&c->partial, sizeof(double) * std::max<int64_t>(reduce_partial_blocks() * 16, 2 * g.nxm + 8)));
   TRYH(hipMalloc(&c->out, sizeof(double) * (2 * g.nxm + 16)));
   c->bytes_held += sizeof(double) * (std::max<int64_t>(reduce_partial_blocks() * 16, 2 * g.nxm + 8) + 2 * g.nxm + 16);
@@ -1049,7 +1054,10 @@ int afid_pressure_spectrum(afid_ctx *c, int stage, double *out) {
   return AFID_OK;
 }
 
-static int correct(afid_ctx *c, int which, const char *name, int ns = 0, bool dev = false) {
+// from_ustar: the velocities before the correction are u* in the b buffers
+// (substep path); otherwise the fields are corrected in place.
+static int correct(afid_ctx *c, int which, const char *name, int ns = 0, bool dev = false, bool from_ustar = false,
+                   bool cfl_fold = false) {
   if (!c) return AFID_ERR_ARG;
   CorrectArgs a;
   a.ds = dev ? c->dstep : nullptr;
@@ -1057,8 +1065,12 @@ static int correct(afid_ctx *c, int which, const char *name, int ns = 0, bool de
   a.g = c->g;
   a.dph = c->f[AFID_DPH];
   a.vx = c->f[AFID_VX]; a.vy = c->f[AFID_VY]; a.vz = c->f[AFID_VZ]; a.pr = c->f[AFID_PR];
+  a.ux = from_ustar ? c->bp[0] : a.vx;
+  a.uy = from_ustar ? c->bp[1] : a.vy;
+  a.uz = from_ustar ? c->bp[2] : a.vz;
   a.kc = c->kc;
   a.sc = scalars(c);
+  a.cflbits = cfl_fold ? &c->dstep->cflbits : nullptr;
   Timed t(c, name);
   return kerr(c, launch_correct(a, which, c->st));
 }
@@ -1076,11 +1088,7 @@ static FusedArgs update_args(afid_ctx *c, int ns, bool dev) {
   FusedArgs a;
   a.g = c->g;
   a.f = fields(c);
-  if (c->split) {   // the column solve reads only b' and vz(k=1): the fields are updated in place
-    a.vxn = c->f[AFID_VX]; a.vyn = c->f[AFID_VY]; a.vzn = c->f[AFID_VZ]; a.tn = c->f[AFID_TEMP];
-  } else {
-    a.vxn = c->fn[0]; a.vyn = c->fn[1]; a.vzn = c->fn[2]; a.tn = c->fn[3];
-  }
+  a.tn = c->f[AFID_TEMP];   // the column solve reads only b' and vz(k=1)
   a.rux = c->f[AFID_RUX]; a.ruy = c->f[AFID_RUY]; a.ruz = c->f[AFID_RUZ]; a.rut = c->f[AFID_RUTEMP];
   a.stale = c->stale;
   a.kc = c->kc;
@@ -1098,28 +1106,21 @@ static FusedArgs update_args(afid_ctx *c, int ns, bool dev) {
   return a;
 }
 
-static int enqueue_substep(afid_ctx *c, int ns, bool dev) {
+// cfl_fold: the correction also reduces CalcMaxCFL into the device step
+// state for the next step (the last substep of a device step).
+static int enqueue_substep(afid_ctx *c, int ns, bool dev, bool cfl_fold = false) {
   int rc;
   FusedArgs a = update_args(c, ns, dev);
-  if (c->split) {
-    {
-      Timed t(c, "stencil_rhs");
-      if ((rc = kerr(c, launch_stencil_rhs(a, c->st)))) return rc;
-    }
+  {
+    Timed t(c, "stencil_rhs");
+    if ((rc = kerr(c, launch_stencil_rhs(a, c->st)))) return rc;
+  }
+  {
     Timed t(c, "column_solve");
     if ((rc = kerr(c, launch_column_solve(a, c->st)))) return rc;
-  } else {
-    {
-      Timed t(c, "fused_update");
-      if ((rc = kerr(c, launch_fused_update(a, c->st)))) return rc;
-    }
-    std::swap(c->f[AFID_VX], c->fn[0]);
-    std::swap(c->f[AFID_VY], c->fn[1]);
-    std::swap(c->f[AFID_VZ], c->fn[2]);
-    std::swap(c->f[AFID_TEMP], c->fn[3]);
   }
-  if ((rc = run_poisson(c, 1, ns, dev))) return rc;
-  return correct(c, 3, "correct_vel_pr", ns, dev);
+  if ((rc = run_poisson(c, 1, ns, dev, true))) return rc;
+  return correct(c, 3, "correct_vel_pr", ns, dev, true, cfl_fold);
 }
 
 int afid_substep(afid_ctx *c, int ns, double dt) {
@@ -1231,10 +1232,17 @@ static int ensure_dev_step(afid_ctx *c) {
 
 // One main.F90 step entirely on the stream: CalcMaxCFL, dt and substep
 // scalars, factorisations, the substeps, time += dt.
+// In FP64 the CFL number comes from the last correction of the previous step
+// (dev_steps_prepare seeds it for the first); the mixed mode's refinement
+// sweep corrects again in place after that, so there k_red_cfl runs first.
 static int enqueue_step_device(afid_ctx *c) {
   int rc;
-  if ((rc = kerr(c, launch_reduce_cfl(reduce_args(c), c->st)))) return rc;
-  if ((rc = kerr(c, launch_step_begin(c->dstep, c->out, c->kconst, c->st)))) return rc;
+  if (c->mixed) {
+    if ((rc = kerr(c, launch_reduce_cfl(reduce_args(c), c->st)))) return rc;
+    if ((rc = kerr(c, launch_step_begin(c->dstep, c->out, c->kconst, c->st)))) return rc;
+  } else if ((rc = kerr(c, launch_step_begin(c->dstep, nullptr, c->kconst, c->st)))) {
+    return rc;
+  }
   FactorArgs fa;
   fa.g = c->g;
   fa.mc = c->mc;
@@ -1244,7 +1252,7 @@ static int enqueue_step_device(afid_ctx *c) {
   fa.k = c->kconst;
   if ((rc = kerr(c, launch_factor(fa, c->st)))) return rc;
   for (int ns = 1; ns <= c->kconst.nsst; ns++)
-    if ((rc = enqueue_substep(c, ns, true))) return rc;
+    if ((rc = enqueue_substep(c, ns, true, !c->mixed && ns == c->kconst.nsst))) return rc;
   if (c->mixed && (rc = refine_projection(c, c->kconst.nsst, true))) return rc;
   return kerr(c, launch_step_end(c->dstep, c->st));
 }
@@ -1257,6 +1265,12 @@ static int dev_steps_prepare(afid_ctx *c, int ntime0, int idtv, double dt, doubl
   DevStep h{};
   h.time = c->time; h.ntime = ntime0; h.idtv = idtv; h.dtfix = dt; h.dtmin = dtmin; h.dtmax = dtmax;
   h.limitcfl = limitcfl; h.dt = dt;
+  if (!c->mixed) {   // CalcMaxCFL of the current fields for the first step
+    if ((rc = kerr(c, launch_reduce_cfl(reduce_args(c), c->st)))) return rc;
+    double v;
+    if ((rc = read_out(c, &v, 1))) return rc;
+    std::memcpy(&h.cflbits, &v, sizeof v);
+  }
   HIPCHK(c, hipMemcpyAsync(c->dstep, &h, sizeof(DevStep), hipMemcpyHostToDevice, c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));
   if (!c->step_graph) {   // capture one step once; its kernels take no per-step host data
@@ -1299,29 +1313,28 @@ int afid_steps(afid_ctx *c, int ntime0, int nsteps, int idtv, double dt, double 
                double limitcfl, double *dt_last, int *errorcode, int *steps_done) {
   if (!c || nsteps < 0 || (!idtv && dt <= 0)) return AFID_ERR_ARG;
   if (c->singular_mode) return AFID_NUM_SINGULAR;
-  if (!c->split) {   // the fused update swaps buffers per substep: step from the host
-    int err = 0, done = 0;
-    double dtu = dt;
-    for (int n = 0; n < nsteps && err == 0; n++, done++) {
-      int rc = afid_step(c, ntime0 + n, idtv, dt, dtmin, dtmax, limitcfl, &dtu, &err);
-      if (rc) return rc;
-    }
-    if (dt_last) *dt_last = dtu;
-    if (errorcode) *errorcode = err;
-    if (steps_done) *steps_done = done;
-    return AFID_OK;
-  }
   int rc = dev_steps_prepare(c, ntime0, idtv, dt, dtmin, dtmax, limitcfl);
   if (!rc) rc = dev_steps_enqueue(c, nsteps);
   if (rc) return rc;
   return dev_steps_finish(c, dt_last, errorcode, steps_done);
 }
 
+int afid_steps_cfl(afid_ctx *c, double *cflm) {
+  if (!c || !cflm) return AFID_ERR_ARG;
+  if (!c->dev_steps_ready || c->mixed) return AFID_ERR_STATE;
+  DevStep h;
+  HIPCHK(c, hipMemcpyAsync(&h, c->dstep, sizeof(DevStep), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  double v;
+  std::memcpy(&v, &h.cflbits, sizeof v);
+  *cflm = std::max(0.00000001, v);
+  return AFID_OK;
+}
+
 int afid_steps_async(afid_ctx *c, int ntime0, int nsteps, int idtv, double dt, double dtmin, double dtmax,
                      double limitcfl) {
   if (!c || nsteps < 0 || (!idtv && dt <= 0)) return AFID_ERR_ARG;
   if (c->singular_mode) return AFID_NUM_SINGULAR;
-  if (!c->split) return AFID_ERR_STATE;
   int rc = dev_steps_prepare(c, ntime0, idtv, dt, dtmin, dtmax, limitcfl);
   return rc ? rc : dev_steps_enqueue(c, nsteps);
 }
@@ -1372,10 +1385,6 @@ int afid_batch_steps(afid_ctx *const *cells, int ncells, int nsteps) {
 // cleared afterwards, so the state is to be set after this call.
 int afid_tune(afid_ctx *c, int *zm_out) {
   if (!c) return AFID_ERR_ARG;
-  if (!c->split) {   // only the split update has the stencil pass
-    if (zm_out) *zm_out = 0;
-    return AFID_OK;
-  }
   static const int cands[] = {8, 16, 32, 64};
   hipEvent_t e0, e1;
   HIPCHK(c, hipEventCreate(&e0));
@@ -1423,7 +1432,9 @@ int afid_estimate_bytes(const afid_params *p, int64_t *bytes) {
   const int64_t field = 8 * ncol * ldx, fac = 8 * (int64_t)SYS_N * FR_N * ldx;
   int64_t b = 14 * field;                                   // 10 fields + 4 b' (or 4 ping-pong) arrays
   b += 8 * ncol;                                            // stale row
-  b += nxm * nzm * ldm * (16 + 8);                          // spectrum and x-solve scratch
+  int np, lm;
+  xsolve_shape((int)nxm, &np, &lm);
+  b += nxm * nzm * ldm * 16 + 8 * (nym / 2 + 1) * nzm * np;  // spectrum, x-solve pivot table
   b += 8 * (int64_t)KC_N * ldx + 8 * (2 * nx + 7 * nxm + nym + nzm);   // coefficient rows
   b += 4 * fac + 3 * fac + 8 * (int64_t)SYS_N * 6 * nx + 8 * 7 * nx + (int64_t)sizeof(DevStep);
   b += 20 * (nym + nzm) + 16 * nym;                         // FFT tables

@@ -13,7 +13,8 @@
 //
 // Spectral array (pressure solve): S[k][n][m] complex128, k in [0,nxm),
 // n in [0,nzm) (z wavenumber), m in [0,ldm) (y wavenumber, m<nym/2+1 used),
-// m fastest.  x-solves walk k with one thread per (m,n) mode.
+// m fastest.  The x-solve holds each (m,n) mode's line in the registers of
+// a group of lanes, one chunk of consecutive k per lane.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -97,6 +98,7 @@ struct DevStep {
   double dt, time, dtfix, dtmin, dtmax, limitcfl;
   int ntime, idtv, errorcode, halted, pivot, steps_done;
   StepScalars sc[3];   // scalars of substeps 1..3 of the current step
+  unsigned long long cflbits;   // CalcMaxCFL for the next step (bits of a non-negative double)
 };
 
 // Host-set constants the device step control needs to rebuild StepScalars.

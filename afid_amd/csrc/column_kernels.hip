@@ -202,64 +202,9 @@ __device__ __forceinline__ double chunk_solve(double *bl, const double *fac, int
   return scan_down(cnt > 0 ? fh[p] : 1.0, cnt > 0 ? acc : 0.0, p);
 }
 
-// Register-resident variant for a compile-time chunk bound LM >= L: the chunk
-// is read from LDS into b[], each sweep is unrolled so that all of its
-// coefficient loads are issued together (the recurrences then run on
-// registers), and b[] returns x without the last carry fix-up, which the
-// caller folds into its write-out: x = b[t] + H_t*cin.
-template <int LM>
-__device__ __forceinline__ double chunk_solve_reg(double (&b)[LM], const double *bl, const double *fac, int ldx,
-                                                  int cnt, int p) {
-  const double *fl = fac + FR_L * ldx, *fr = fac + FR_RINV * ldx, *fe = fac + FR_E * ldx;
-  const double *fg = fac + FR_G * ldx, *fh = fac + FR_H * ldx;
-#pragma unroll
-  for (int t = 0; t < LM; t++) b[t] = t < cnt ? bl[t * WG] : 0.0;
-  double acc = 0.0;
-#pragma unroll
-  for (int t = 0; t < LM; t++) {
-    if (t < cnt) {
-      acc = b[t] - fl[t * P + p] * acc;
-      b[t] = acc;
-    }
-  }
-  const double cin = scan_up(cnt > 0 ? fg[(cnt - 1) * P + p] : 1.0, cnt > 0 ? acc : 0.0, p);
-  acc = 0.0;
-#pragma unroll
-  for (int t = LM - 1; t >= 0; t--) {
-    if (t < cnt) {
-      const int s = t * P + p;
-      const double y = b[t] + fg[s] * cin;
-      acc = y * fr[s] - fe[s] * acc;
-      b[t] = acc;
-    }
-  }
-  return scan_down(cnt > 0 ? fh[p] : 1.0, cnt > 0 ? acc : 0.0, p);
-}
-
-// Positions of k-1 / k+1 for lane p, step t (valid only when those k exist).
-__device__ __forceinline__ int lane_skm(int p, int t, int L) { return t > 0 ? (t - 1) * P + p : (L - 1) * P + p - 1; }
-__device__ __forceinline__ int lane_skp(int p, int t, int L) { return t < L - 1 ? (t + 1) * P + p : p + 1; }
-
-// ---------------------------------------------------------------- fused substep
-// One pass per column over all four prognostic fields: explicit terms, RHS,
-// x-implicit solve and update, with the new fields written to separate
-// buffers (explicit terms of all fields read the old ones, TimeMarcher.F90:32-40).
-// rhs(1,j,i) of the reference (stale plane) is read by the vx solve and
-// replaced by the vz increment at k=1 (SolveImpEqnUpdate_YZ.F90:50), which
-// the temperature solve of the same substep then reads.
-//
-// Layout of the work: a block owns a 4x4 tile of columns (wave w = z row
-// i0+w, four y columns per wave, 16 chunk lanes per column).  Tiles are
-// visited in strips 8 tiles wide in y, z fastest, and each XCD gets one
-// contiguous range of that order, so the i+-1 neighbour columns a tile reads
-// were touched a few tiles earlier on the same XCD and come from its L2.
-// Each lane walks its chunk in k keeping the k-1/k/k+1 values of the centre
-// column in registers.  The k loops carry no branches: every position a lane
-// reads lies inside the column's ldx slots (padding slots hold zeros), values
-// are computed for boundary rows too and discarded by selects, so the loads
-// of consecutive k steps overlap.  The system is solved for the new value
-// directly: A q_new = b + A q_old, with A the unit-diagonal normalised matrix,
-// which equals q_old + A^-1 b of SolveImpEqnUpdate_* and spares re-reading q_old.
+// ---------------------------------------------------------------- load/store helpers
+// Byte-offset loads and stores against a uniform base pointer (32-bit offsets
+// while a field fits in 4 GiB).
 template <typename OffT>
 __device__ __forceinline__ double ldf(const double *base, OffT boff) {
   return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(base) + boff);
@@ -281,268 +226,11 @@ __device__ __forceinline__ void sth(double *base, OffT boff, double v) {
   else stf(base, boff, v);
 }
 
-template <typename OffT>
-struct ColB {
-  OffT C, CJM, CJP, CIM, CIP, CJMIP, CJPIM;
-};
-
-constexpr int TILE_STRIP = 8;
-
-// The field pointers are separate __restrict__ parameters: the history stores
-// inside the k loops then cannot alias the stencil loads.
-template <int LM, typename OffT, bool USE_RU>
-__global__ void __launch_bounds__(WG)
-    k_fused_update(FusedArgs a, const double *__restrict__ vx, const double *__restrict__ vy,
-                   const double *__restrict__ vz, const double *__restrict__ T, const double *__restrict__ pr,
-                   double *__restrict__ rux, double *__restrict__ ruy, double *__restrict__ ruz,
-                   double *__restrict__ rut, double *__restrict__ vxn, double *__restrict__ vyn,
-                   double *__restrict__ vzn, double *__restrict__ tn, double *__restrict__ stalep,
-                   const double *__restrict__ kc, const double *__restrict__ facp) {
-  extern __shared__ double lds[];
-  const Geom g = a.g;
-  const int ntj = (g.nym + 3) / 4, nti = (g.nzm + 3) / 4;
-  const int64_t lin = xcd_block((int64_t)ntj * nti);
-  if (lin < 0) return;
-  const int64_t strip = lin / ((int64_t)TILE_STRIP * nti);
-  const int64_t rr = lin - strip * TILE_STRIP * nti;
-  const int sw = min(TILE_STRIP, ntj - (int)strip * TILE_STRIP);
-  const int ti = (int)(rr / sw), tj = (int)strip * TILE_STRIP + (int)(rr % sw);
-  const int lane = threadIdx.x % 64;
-  const int p = lane % P;
-  const int j = 4 * tj + lane / P, i = 4 * ti + (int)threadIdx.x / 64;
-  const bool live = j < g.nym && i < g.nzm;
-  const int jc = live ? j : 0, ic = live ? i : 0;
-  const int jm = wrapm(jc - 1, g.nym), jp = wrapp(jc + 1, g.nym);
-  const int im = wrapm(ic - 1, g.nzm), ip = wrapp(ic + 1, g.nzm);
-  const OffT cb = (OffT)g.ldx * sizeof(double);
-  auto colb = [&](int jj, int ii) { return ((OffT)ii * (OffT)g.nym + (OffT)jj) * cb; };
-  ColB<OffT> n;
-  n.C = colb(jc, ic); n.CJM = colb(jm, ic); n.CJP = colb(jp, ic); n.CIM = colb(jc, im); n.CIP = colb(jc, ip);
-  n.CJMIP = colb(jm, ip); n.CJPIM = colb(jp, im);
-  const int64_t colc = (int64_t)ic * g.nym + jc;
-  const int L = g.L, ldx = g.ldx, nx = g.nx, nxm = g.nxm;
-  const StepScalars &c = a.sc;
-  const OffT rowb = (OffT)ldx * sizeof(double);             // one coefficient row in bytes
-  const double *kcX = kc + 0;                               // row q at kc + q*ldx
-  double *bl = lds + threadIdx.x;
-  const int k0 = p * L;
-  const int cntx = live ? max(0, min(L, nx - k0)) : 0;    // points of an nx system
-  const int cntm = live ? max(0, min(L, nxm - k0)) : 0;   // points of an nxm system
-  const OffT smb = (OffT)(k0 >= 1 ? lane_skm(p, 0, L) : p) * sizeof(double);
-  const OffT s0b = (OffT)p * sizeof(double);
-  // coefficient row byte offsets
-  const OffT rUDX3C = KC_UDX3C * rowb, rUDX3M = KC_UDX3M * rowb, rAM3CK = KC_AM3CK * rowb,
-             rAC3CK = KC_AC3CK * rowb, rAP3CK = KC_AP3CK * rowb, rAM3SK = KC_AM3SK * rowb,
-             rAC3SK = KC_AC3SK * rowb, rAP3SK = KC_AP3SK * rowb;
-  const OffT fSC = FR_SCALE * rowb, fDL = FR_DL * rowb, fDU = FR_DU * rowb;
-
-  // ---------------- vx: ImplicitAndUpdateVX + SolveImpEqnUpdate_X (n = nx)
-  {
-    const double *fac = facp + (SYS_X * FR_N) * ldx;
-    double vxc = ldf(vx, n.C + s0b);
-    double vxm = ldf(vx, n.C + smb), vym = ldf(vy, n.C + smb), vyjpm = ldf(vy, n.CJP + smb);
-    double vzm = ldf(vz, n.C + smb), vzipm = ldf(vz, n.CIP + smb), prm = ldf(pr, n.C + smb);
-    const double stale = (live && p == 0) ? stalep[colc] : 0.0;
-#pragma unroll 2
-    for (int t = 0; t < cntx; t++) {
-      const int k = k0 + t, s = t * P + p;
-      const OffT sb = (OffT)s * sizeof(double), spb = (OffT)lane_skp(p, t, L) * sizeof(double);
-      const double vxp = ldf(vx, n.C + spb);
-      const double vyc = ldf(vy, n.C + sb), vyjpc = ldf(vy, n.CJP + sb);
-      const double vzc = ldf(vz, n.C + sb), vzipc = ldf(vz, n.CIP + sb), prc = ldf(pr, n.C + sb);
-      const double vxip = ldf(vx, n.CIP + sb), vxim = ldf(vx, n.CIM + sb);
-      const double vxjp = ldf(vx, n.CJP + sb), vxjm = ldf(vx, n.CJM + sb);
-      const double tcur = ldf(T, n.C + sb);
-      const double udx3c = ldf(kcX, rUDX3C + sb);
-      const double am = ldf(kcX, rAM3CK + sb), ac = ldf(kcX, rAC3CK + sb), ap = ldf(kcX, rAP3CK + sb);
-      const double scl = ldf(fac, fSC + sb), dl = ldf(fac, fDL + sb), du = ldf(fac, fDU + sb);
-      const double ru = USE_RU ? ldf(rux, n.C + sb) : 0.0;
-      double hxz = (((vzipc + vzipm) * (vxip + vxc)) - ((vzc + vzm) * (vxc + vxim))) * c.udz4;
-      double hxy = (((vyjpc + vyjpm) * (vxjp + vxc)) - ((vyc + vym) * (vxc + vxjm))) * c.udy4;
-      double aa = vxp + vxc, bb = vxc + vxm;
-      double hxx = (aa * aa - bb * bb) * udx3c * 0.25;
-      double dzzvx = (vxim - 2.0 * vxc + vxip) * c.udzq_re;
-      double dyyvx = (vxjm - 2.0 * vxc + vxjp) * c.udyq_re;
-      double q = -(hxx + hxy + hxz) + dyyvx + dzzvx + tcur;
-      double dxxvx = vxp * ap + vxc * ac + vxm * am;
-      double dxp = (prc - prm) * (c.al * udx3c);
-      double r = USE_RU ? c.ro * ru : 0.0;
-      double rhs = (c.ga * q + r + c.alre * dxxvx - dxp) * c.dt;
-      const bool interior = k >= 1 && k <= nx - 2;
-      if (interior && a.keep_ru) stf(rux, n.C + sb, q);
-      double vin = rhs * scl + ((dl * vxm + vxc) + du * vxp);
-      double vbd = (k == 0 ? stale : 0.0) + vxc;  // identity rows: rhs(1) stale, rhs(nx) zero
-      bl[t * WG] = interior ? vin : vbd;
-      vxm = vxc; vxc = vxp; vym = vyc; vyjpm = vyjpc; vzm = vzc; vzipm = vzipc; prm = prc;
-    }
-    double b[LM];
-    const double cin = chunk_solve_reg<LM>(b, bl, fac, ldx, cntx, p);
-    const double *fh = fac + FR_H * ldx;
-#pragma unroll
-    for (int t = 0; t < LM; t++) {
-      if (t < cntx) {
-        const int k = k0 + t, s = t * P + p;
-        const double y = b[t] + fh[s] * cin;
-        stf(vxn, n.C + (OffT)s * sizeof(double), (k >= 1 && k <= nx - 2) ? y : 0.0);
-      }
-    }
-  }
-  // ---------------- vy: ImplicitAndUpdateVY + SolveImpEqnUpdate_YZ (n = nxm)
-  {
-    const double *fac = facp + (SYS_YZ * FR_N) * ldx;
-    double vxk = ldf(vx, n.C + s0b), vxjmk = ldf(vx, n.CJM + s0b), vyc = ldf(vy, n.C + s0b);
-    double vym = k0 >= 1 ? ldf(vy, n.C + smb) : vyc;
-#pragma unroll 2
-    for (int t = 0; t < cntm; t++) {
-      const int k = k0 + t, s = t * P + p;
-      const OffT sb = (OffT)s * sizeof(double), spb = (OffT)lane_skp(p, t, L) * sizeof(double);
-      const double vxkp = ldf(vx, n.C + spb), vxjmkp = ldf(vx, n.CJM + spb);
-      const double vynext = ldf(vy, n.C + spb);
-      const double vyp = k == nxm - 1 ? vyc : vynext;               // kpv clamp
-      const double vymm = k == 0 ? vyc : vym;                       // kmv clamp
-      const double vyjp = ldf(vy, n.CJP + sb), vyjm = ldf(vy, n.CJM + sb);
-      const double vyip = ldf(vy, n.CIP + sb), vyim = ldf(vy, n.CIM + sb);
-      const double vzip = ldf(vz, n.CIP + sb), vzjmip = ldf(vz, n.CJMIP + sb);
-      const double vzc = ldf(vz, n.C + sb), vzjm = ldf(vz, n.CJM + sb);
-      const double prc = ldf(pr, n.C + sb), prjm = ldf(pr, n.CJM + sb);
-      const double udx3m = ldf(kcX, rUDX3M + sb);
-      const double am = ldf(kcX, rAM3SK + sb), ac = ldf(kcX, rAC3SK + sb), ap = ldf(kcX, rAP3SK + sb);
-      const double scl = ldf(fac, fSC + sb), dl = ldf(fac, fDL + sb), du = ldf(fac, fDU + sb);
-      const double ru = USE_RU ? ldf(ruy, n.C + sb) : 0.0;
-      double hyx = ((vxkp + vxjmkp) * (vyp + vyc) - (vxk + vxjmk) * (vyc + vymm)) * udx3m * 0.25;
-      double yp = vyjp + vyc, ym = vyjm + vyc;
-      double hyy = (yp * yp - ym * ym) * c.udy4;
-      double hyz = ((vyip + vyc) * (vzip + vzjmip) - (vyc + vyim) * (vzc + vzjm)) * c.udz4;
-      double dyyvy = (vyjp - 2.0 * vyc + vyjm) * c.udyq_re;
-      double dzzvy = (vyip - 2.0 * vyc + vyim) * c.udzq_re;
-      double h = -(hyx + hyy + hyz) + dyyvy + dzzvy;
-      double dxxvy = vyp * ap + vyc * ac + vymm * am;
-      double dyp = (prc - prjm) * c.udy_al;
-      double r = USE_RU ? c.ro * ru : 0.0;
-      double rhs = (c.ga * h + r + c.alre * dxxvy - dyp) * c.dt;
-      if (a.keep_ru) stf(ruy, n.C + sb, h);
-      bl[t * WG] = rhs * scl + ((dl * vym + vyc) + du * vyp);
-      vym = vyc; vyc = vynext; vxk = vxkp; vxjmk = vxjmkp;
-    }
-    double b[LM];
-    const double cin = chunk_solve_reg<LM>(b, bl, fac, ldx, cntm, p);
-    const double *fh = fac + FR_H * ldx;
-#pragma unroll
-    for (int t = 0; t < LM; t++) {
-      if (t < cntx) {
-        const int k = k0 + t, s = t * P + p;
-        // row nx is outside the nxm system: the reference never writes it
-        stf(vyn, n.C + (OffT)s * sizeof(double), k < nxm ? b[t] + fh[s] * cin : ldf(vy, n.C + (OffT)s * sizeof(double)));
-      }
-    }
-  }
-  // ---------------- vz: ImplicitAndUpdateVZ + SolveImpEqnUpdate_YZ (n = nxm)
-  double dvz0 = 0.0;
-  {
-    const double *fac = facp + (SYS_YZ * FR_N) * ldx;
-    double vxk = ldf(vx, n.C + s0b), vximk = ldf(vx, n.CIM + s0b), vzc = ldf(vz, n.C + s0b);
-    double vzm = k0 >= 1 ? ldf(vz, n.C + smb) : vzc;
-    const double vz0 = vzc;
-#pragma unroll 2
-    for (int t = 0; t < cntm; t++) {
-      const int k = k0 + t, s = t * P + p;
-      const OffT sb = (OffT)s * sizeof(double), spb = (OffT)lane_skp(p, t, L) * sizeof(double);
-      const double vxkp = ldf(vx, n.C + spb), vximkp = ldf(vx, n.CIM + spb);
-      const double vznext = ldf(vz, n.C + spb);
-      const double vzp = k == nxm - 1 ? vzc : vznext;
-      const double vzmm = k == 0 ? vzc : vzm;
-      const double vzip = ldf(vz, n.CIP + sb), vzim = ldf(vz, n.CIM + sb);
-      const double vzjp = ldf(vz, n.CJP + sb), vzjm = ldf(vz, n.CJM + sb);
-      const double vyjp = ldf(vy, n.CJP + sb), vyjpim = ldf(vy, n.CJPIM + sb);
-      const double vyc = ldf(vy, n.C + sb), vyim = ldf(vy, n.CIM + sb);
-      const double prc = ldf(pr, n.C + sb), prim = ldf(pr, n.CIM + sb);
-      const double udx3m = ldf(kcX, rUDX3M + sb);
-      const double am = ldf(kcX, rAM3SK + sb), ac = ldf(kcX, rAC3SK + sb), ap = ldf(kcX, rAP3SK + sb);
-      const double scl = ldf(fac, fSC + sb), dl = ldf(fac, fDL + sb), du = ldf(fac, fDU + sb);
-      const double ru = USE_RU ? ldf(ruz, n.C + sb) : 0.0;
-      double zp = vzip + vzc, zm = vzim + vzc;
-      double hzz = (zp * zp - zm * zm) * c.udz4;
-      double hzy = ((vyjp + vyjpim) * (vzjp + vzc) - (vyc + vyim) * (vzc + vzjm)) * c.udy4;
-      double hzx = ((vxkp + vximkp) * (vzp + vzc) - (vxk + vximk) * (vzc + vzmm)) * udx3m * 0.25;
-      double dzzvz = (vzip - 2.0 * vzc + vzim) * c.udzq_re;
-      double dyyvz = (vzjp - 2.0 * vzc + vzjm) * c.udyq_re;
-      double h = -(hzx + hzy + hzz) + dyyvz + dzzvz;
-      double dxxvz = vzp * ap + vzc * ac + vzmm * am;
-      double dzp = (prc - prim) * c.dz * c.al;
-      double r = USE_RU ? c.ro * ru : 0.0;
-      double rhs = (c.ga * h + r + c.alre * dxxvz - dzp) * c.dt;
-      if (a.keep_ru) stf(ruz, n.C + sb, h);
-      bl[t * WG] = rhs * scl + ((dl * vzm + vzc) + du * vzp);
-      vzm = vzc; vzc = vznext; vxk = vxkp; vximk = vximkp;
-    }
-    double b[LM];
-    const double cin = chunk_solve_reg<LM>(b, bl, fac, ldx, cntm, p);
-    const double *fh = fac + FR_H * ldx;
-#pragma unroll
-    for (int t = 0; t < LM; t++) {
-      if (t < cntx) {
-        const int k = k0 + t, s = t * P + p;
-        const double y = k < nxm ? b[t] + fh[s] * cin : ldf(vz, n.C + (OffT)s * sizeof(double));
-        if (t == 0 && k == 0) dvz0 = y - vz0;
-        stf(vzn, n.C + (OffT)s * sizeof(double), y);
-      }
-    }
-  }
-  dvz0 = __shfl(dvz0, 0, P);  // lane p=0 of the column holds k=0
-  if (live && p == 0) stalep[colc] = dvz0;
-  // ---------------- temperature: SolveImpEqnUpdate_Temp (n = nx), rhs(1) = vz increment at k=1
-  {
-    const double *fac = facp + (SYS_T * FR_N) * ldx;
-    double tc = ldf(T, n.C + s0b), vxc = ldf(vx, n.C + s0b);
-    double tm = ldf(T, n.C + smb), vxm = ldf(vx, n.C + smb), vzm = ldf(vz, n.C + smb);
-    double vzipm = ldf(vz, n.CIP + smb), vym = ldf(vy, n.C + smb), vyjpm = ldf(vy, n.CJP + smb);
-#pragma unroll 2
-    for (int t = 0; t < cntx; t++) {
-      const int k = k0 + t, s = t * P + p;
-      const OffT sb = (OffT)s * sizeof(double), spb = (OffT)lane_skp(p, t, L) * sizeof(double);
-      const double tp = ldf(T, n.C + spb), vxp = ldf(vx, n.C + spb);
-      const double vzipc = ldf(vz, n.CIP + sb), vzc = ldf(vz, n.C + sb);
-      const double vyjpc = ldf(vy, n.CJP + sb), vyc = ldf(vy, n.C + sb);
-      const double tip = ldf(T, n.CIP + sb), tim = ldf(T, n.CIM + sb);
-      const double tjp = ldf(T, n.CJP + sb), tjm = ldf(T, n.CJM + sb);
-      const double udx3c = ldf(kcX, rUDX3C + sb);
-      const double am = ldf(kcX, rAM3CK + sb), ac = ldf(kcX, rAC3CK + sb), ap = ldf(kcX, rAP3CK + sb);
-      const double scl = ldf(fac, fSC + sb), dl = ldf(fac, fDL + sb), du = ldf(fac, fDU + sb);
-      const double ru = USE_RU ? ldf(rut, n.C + sb) : 0.0;
-      double htz = ((vzipm + vzipc) * (tip + tc) - (vzm + vzc) * (tc + tim)) * c.udz4;
-      double hty = ((vyjpc + vyjpm) * (tjp + tc) - (vyc + vym) * (tc + tjm)) * c.udy4;
-      double htx = ((vxp + vxc) * (tp + tc) - (vxc + vxm) * (tc + tm)) * udx3c * 0.25;
-      double dzzt = (tip - 2.0 * tc + tim) * c.udzq_pe;
-      double dyyt = (tjp - 2.0 * tc + tjm) * c.udyq_pe;
-      double h = -(htx + hty + htz) + dyyt + dzzt;
-      double dxxt = tp * ap + tc * ac + tm * am;
-      double r = USE_RU ? c.ro * ru : 0.0;
-      double rhs = (c.ga * h + r + c.alpec * dxxt) * c.dt;
-      const bool interior = k >= 1 && k <= nx - 2;
-      if (interior && a.keep_ru) stf(rut, n.C + sb, h);
-      double vin = rhs * scl + ((dl * tm + tc) + du * tp);
-      double vbd = (k == 0 ? dvz0 : 0.0) + tc;
-      bl[t * WG] = interior ? vin : vbd;
-      tm = tc; tc = tp; vxm = vxc; vxc = vxp; vzm = vzc; vzipm = vzipc; vym = vyc; vyjpm = vyjpc;
-    }
-    double b[LM];
-    const double cin = chunk_solve_reg<LM>(b, bl, fac, ldx, cntx, p);
-    const double *fh = fac + FR_H * ldx;
-#pragma unroll
-    for (int t = 0; t < LM; t++) {
-      if (t < cntx) {
-        const int k = k0 + t, s = t * P + p;
-        double y = b[t] + fh[s] * cin;
-        if (k == 0) y = c.tempbp;
-        if (k == nx - 1) y = c.temptp;
-        stf(tn, n.C + (OffT)s * sizeof(double), y);
-      }
-    }
-  }
-}
-
-// ---------------------------------------------------------------- split update
-// The same substep update as k_fused_update in two passes.
+// ---------------------------------------------------------------- substep update
+// The explicit terms, right-hand sides and wall-normal solves of one substep
+// in two passes.  Each system is solved for the new value directly:
+// A q_new = b + A q_old, with A the unit-diagonal normalised matrix, which
+// equals q_old + A^-1 b of SolveImpEqnUpdate_* and spares re-reading q_old.
 // k_stencil_rhs: the explicit terms and right-hand sides of all four fields
 // (ExplicitTerms*, ImplicitAndUpdate* RHS loops), the history update, and
 // b' = ackl_b*rhs + A q_old, where A is the unit-diagonal normalised system of
@@ -562,8 +250,12 @@ __global__ void __launch_bounds__(WG)
 // substep 3, since the next step's substep 1 has ro = 0) are not stored.
 // k_column_solve: a wave owns four columns (16 chunk lanes each) and solves the
 // four systems in registers: A q_new = b', with the factor rows staged in LDS.
-// The vz increment at k=1, q_new - q_old, becomes the new stale row and the
-// temperature's rhs(1).
+// The new velocities u* (before the pressure correction) overwrite b' in the
+// b buffers, each lane rewriting only the positions it read; the divergence
+// and the correction read u* there, and the correction writes the fields, so
+// the fields keep the substep's starting values until then.  The new
+// temperature is final and goes to its field.  The vz increment at k=1,
+// q_new - q_old, becomes the new stale row and the temperature's rhs(1).
 constexpr int ZM_DEFAULT = 32;   // z planes per block unless afid_tune chose another count
 constexpr int SY_COLS = 4;       // y columns per block, one wave each
 constexpr int SY_WG = SY_COLS * 64;
@@ -839,10 +531,9 @@ static size_t solve_lds(const Geom &g) { return sizeof(double) * SYS_N * RW_N * 
 
 template <int LM, typename OffT>
 __global__ void __launch_bounds__(SOLVE_WG)
-    k_column_solve(FusedArgs a, const double *__restrict__ vz, const double *__restrict__ bx,
-                   const double *__restrict__ by, const double *__restrict__ bz, const double *__restrict__ bt,
-                   double *__restrict__ vxn, double *__restrict__ vyn, double *__restrict__ vzn,
-                   double *__restrict__ tn, double *__restrict__ stalep, const double *__restrict__ facp) {
+    k_column_solve(FusedArgs a, const double *__restrict__ vz, double *__restrict__ bx, double *__restrict__ by,
+                   double *__restrict__ bz, const double *__restrict__ bt, double *__restrict__ tn,
+                   double *__restrict__ stalep, const double *__restrict__ facp) {
   extern __shared__ double rows[];   // [SYS_N][RW_N][ldx]
   const Geom g = a.g;
   const int64_t blk = xcd_block((g.ncol + SOLVE_COLS - 1) / SOLVE_COLS);
@@ -875,7 +566,7 @@ __global__ void __launch_bounds__(SOLVE_WG)
       for (int t = 0; t < LM; t++)
         if (t < L) {   // slots of k >= nx (t >= cntx) are written 0
           const int k = k0 + t, s = t * P + p;
-          stf(vxn, C + (OffT)s * sizeof(double), (k >= 1 && k <= nx - 2) ? b[t] + fh[s] * cin : 0.0);
+          stf(bx, C + (OffT)s * sizeof(double), (k >= 1 && k <= nx - 2) ? b[t] + fh[s] * cin : 0.0);
         }
   }
   load_chunk<LM, OffT>(b, by, C, cntm, L, p);
@@ -887,8 +578,8 @@ __global__ void __launch_bounds__(SOLVE_WG)
       for (int t = 0; t < LM; t++)
         if (t < L) {
           const int k = k0 + t, s = t * P + p;
-          // in place: row nx, outside the nxm system, is left as the reference leaves it
-          if (k != nx - 1) stf(vyn, C + (OffT)s * sizeof(double), k < nxm ? b[t] + fh[s] * cin : 0.0);
+          // row nx is outside the nxm system: nothing reads it from here
+          if (k != nx - 1) stf(by, C + (OffT)s * sizeof(double), k < nxm ? b[t] + fh[s] * cin : 0.0);
         }
   }
   load_chunk<LM, OffT>(b, bz, C, cntm, L, p);
@@ -903,7 +594,7 @@ __global__ void __launch_bounds__(SOLVE_WG)
       for (int t = 0; t < LM; t++)
         if (t < L) {
           const int k = k0 + t, s = t * P + p;
-          if (k != nx - 1) stf(vzn, C + (OffT)s * sizeof(double), k < nxm ? b[t] + fh[s] * cin : 0.0);
+          if (k != nx - 1) stf(bz, C + (OffT)s * sizeof(double), k < nxm ? b[t] + fh[s] * cin : 0.0);
         }
   }
   dvz0 = __shfl(dvz0, 0, P);
@@ -1066,17 +757,30 @@ __global__ void k_divergence(RoutineArgs a) {
 }
 
 // CorrectVelocity.F90:28-45 and CorrectPressure.F90:32-52 from dph (= dphhalo).
-// which: 1 velocity, 2 pressure, 3 both (the fused path).  The block shape is
-// that of k_stencil_rhs: SY_COLS waves on neighbouring y columns at the same
-// 64 stored positions, marching through zplanes z planes with dph of planes
+// which: 1 velocity, 2 pressure, 3 both (the substep path).  The velocities
+// are read from (ux, uy, uz) and written to (vx, vy, vz): the u* buffers of
+// the substep path, or the fields themselves (in place) for the per-routine
+// entries and the refinement sweep.  The block shape is that of
+// k_stencil_rhs: SY_COLS waves on neighbouring y columns at the same 64
+// stored positions, marching through zplanes z planes with dph of planes
 // i-1, i, i+1 in registers; each wave publishes its plane-i dph in LDS for its
 // y neighbours (double-buffered, one barrier per plane), so only the group's
 // edge columns load a y neighbour.  Positions past k = nxm are not written.
-constexpr int CZ_ROWS = 4;   // udx3c, amphk, acphk, apphk
+//
+// CFL = true (the last correction of a device step, not in place) also
+// reduces CalcMaxCFL.F90:26-41 over the corrected velocities into
+// a.cflbits (the bits of a non-negative double, so an integer atomic max is
+// the floating-point max).  The corrected value of each neighbour a point
+// needs (vx at k+1, vy at j+1, vz at i+1) is recomputed from u* and dph by
+// the same fused multiply-add that stores it, so the maximum equals that of
+// k_red_cfl over the stored fields bit for bit.
+constexpr int CZ_ROWS = 6;   // udx3c, amphk, acphk, apphk, udx3m, udx3c at k+1
+__device__ __forceinline__ double corr(double u, double dd, double f) { return __builtin_fma(-dd, f, u); }
+template <bool CFL>
 __global__ void __launch_bounds__(SY_WG) k_correct(CorrectArgs a, int which, int zplanes) {
   extern __shared__ double sh[];
   double *cr = sh;                    // [CZ_ROWS][64]
-  double *xv = sh + CZ_ROWS * 64;     // [2][SY_WG] published dph of plane i
+  double *xv = sh + CZ_ROWS * 64;     // [2][2][SY_WG] published dph (and u*_y) of plane i
   const Geom g = a.g;
   const int L = g.L, ldx = g.ldx, nxm = g.nxm;
   const int nq = (ldx + 63) / 64;
@@ -1090,8 +794,14 @@ __global__ void __launch_bounds__(SY_WG) k_correct(CorrectArgs a, int which, int
   const StepScalars c = a.ds ? a.ds->sc[a.ns - 1] : a.sc;
   for (int idx = threadIdx.x; idx < CZ_ROWS * 64; idx += SY_WG) {
     const int r = idx / 64, sl = min(q * 64 + idx - r * 64, ldx - 1);
-    const int src = r == 0 ? KC_UDX3C : (r == 1 ? KC_AMPHK : (r == 2 ? KC_ACPHK : KC_APPHK));
-    cr[idx] = a.kc[src * ldx + sl];
+    int sp = sl;
+    if (r == 5) {   // the position of k+1 (k+1 <= nx-1 holds for every stored k < nx-1)
+      const int kk = k_of_pos(sl, L);
+      sp = kk + 1 < g.nx ? pos_of_k(kk + 1, L) : sl;
+    }
+    const int src = r == 0 || r == 5 ? KC_UDX3C
+                    : (r == 1 ? KC_AMPHK : (r == 2 ? KC_ACPHK : (r == 3 ? KC_APPHK : KC_UDX3M)));
+    cr[idx] = a.kc[src * ldx + sp];
   }
   __syncthreads();
   const int lane = (int)threadIdx.x & 63;
@@ -1103,49 +813,93 @@ __global__ void __launch_bounds__(SY_WG) k_correct(CorrectArgs a, int which, int
   const int s = min(sraw, ldx - 1);
   const int k = k_of_pos(s, L);
   const bool live = colok && sraw < ldx && k <= nxm - 1;
+  const bool top = colok && sraw < ldx && k == g.nx - 1;   // vx wall row: u* (0) passes to the field
   const bool jp_in = w + 1 < ncols, jm_in = w > 0;
   const int jm = wrapm(jc - 1, g.nym), jp = wrapp(jc + 1, g.nym);
   const int skm = (k == 0 || k > nxm - 1) ? s : pos_of_k(k - 1, L);
   const int skp = (k >= nxm - 1) ? s : pos_of_k(k + 1, L);
+  const int sx1 = k + 1 < g.nx ? pos_of_k(k + 1, L) : s;   // vx at k+1 (the wall row for k = nxm-1)
   const double *__restrict__ dph = a.dph;
-  double *__restrict__ qx = a.vx, *__restrict__ qy = a.vy, *__restrict__ qz = a.vz, *__restrict__ qp = a.pr;
+  const double *ux = a.ux, *uy = a.uy, *uz = a.uz;
+  double *qx = a.vx, *qy = a.vy, *qz = a.vz, *__restrict__ qp = a.pr;
   const double usukm = c.al * c.dt * cr[lane];
   const double udy = c.al * c.dt * c.dy, udz = c.al * c.dt * c.dz;
   const double be = c.al * c.beta;
   const double amm = cr[64 + lane], acc = cr[128 + lane], app = cr[192 + lane];
+  const double u3m = cr[256 + lane], usukp = c.al * c.dt * cr[320 + lane];
   const int i0 = ib * zplanes, i1 = min(i0 + zplanes, g.nzm);
   double dim_ = dph[col_off(jc, wrapm(i0 - 1, g.nzm), g) + s];
   double d = dph[col_off(jc, i0, g) + s];
+  double uzc = CFL ? uz[col_off(jc, i0, g) + s] : 0.0;   // u*_z of the current plane, carried
   const int tid = (int)threadIdx.x;
   int par = 0;
+  double cflm = 0.0;
   for (int ic = i0; ic < i1; ic++) {
     const int64_t C = col_off(jc, ic, g);
-    const double dip = dph[col_off(jc, wrapp(ic + 1, g.nzm), g) + s];
+    const int64_t CIP = col_off(jc, wrapp(ic + 1, g.nzm), g);
+    const double dip = dph[CIP + s];
     const double dkm = dph[C + skm], dkp = dph[C + skp];
     double djm = 0.0, djp = 0.0;
     if (!jm_in) djm = dph[col_off(jm, ic, g) + s];
     if (!jp_in) djp = dph[col_off(jp, ic, g) + s];
     double vxo = 0.0, vyo = 0.0, vzo = 0.0, pro = 0.0;
-    if (which & 1) { vxo = qx[C + s]; vyo = qy[C + s]; vzo = qz[C + s]; }
+    double uxk1 = 0.0, uyjp = 0.0, uzip = 0.0;
+    if (which & 1) {
+      vxo = ux[C + s];
+      vyo = uy[C + s];
+      vzo = CFL ? uzc : uz[C + s];
+    }
+    if (CFL) {
+      uxk1 = ux[C + sx1];
+      uzip = uz[CIP + s];
+      if (!jp_in) uyjp = uy[col_off(jp, ic, g) + s];
+    }
     if (which & 2) pro = qp[C + s];
-    double *buf = xv + par * SY_WG;
+    double *buf = xv + par * (2 * SY_WG);
     buf[tid] = d;
+    if (CFL) buf[SY_WG + tid] = vyo;
     __syncthreads();
     if (jm_in) djm = buf[tid - 64];
     if (jp_in) djp = buf[tid + 64];
+    if (CFL && jp_in) uyjp = buf[SY_WG + tid + 64];
     par ^= 1;
     if (live) {
       if (which & 1) {
-        qx[C + s] = vxo - (d - dkm) * usukm;
-        qy[C + s] = vyo - (d - djm) * udy;
-        qz[C + s] = vzo - (d - dim_) * udz;
+        const double nvx = corr(vxo, d - dkm, usukm);
+        const double nvy = corr(vyo, d - djm, udy);
+        const double nvz = corr(vzo, d - dim_, udz);
+        qx[C + s] = nvx;
+        qy[C + s] = nvy;
+        qz[C + s] = nvz;
+        if (CFL) {
+          const double nvxk1 = k + 1 <= nxm - 1 ? corr(uxk1, dkp - d, usukp) : uxk1;
+          const double nvyjp = corr(uyjp, djp - d, udy);
+          const double nvzip = corr(uzip, dip - d, udz);
+          const double qcf = (fabs((nvz + nvzip) * 0.5 * c.dz) + fabs((nvy + nvyjp) * 0.5 * c.dy) +
+                              fabs((nvx + nvxk1) * 0.5 * u3m));
+          cflm = fmax(cflm, qcf);
+        }
       }
       if (which & 2)
         qp[C + s] = pro + d - be * ((dip - 2.0 * d + dim_) * c.dzq + (djp - 2.0 * d + djm) * c.dyq +
                                     (dkp * app + d * acc + dkm * amm));
     }
+    if (top && (which & 1)) qx[C + s] = vxo;
     dim_ = d;
     d = dip;
+    uzc = uzip;
+  }
+  if (CFL) {   // one atomic per block
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) cflm = fmax(cflm, __shfl_xor(cflm, off, 64));
+    __syncthreads();   // every wave is past its last read of xv
+    if (lane == 0) xv[w] = cflm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double m = xv[0];
+      for (int q = 1; q < SY_COLS; q++) m = fmax(m, xv[q]);
+      atomicMax(a.cflbits, (unsigned long long)__double_as_longlong(m));
+    }
   }
 }
 
@@ -1161,19 +915,8 @@ static int pick_lm(int L) {
   return -1;
 }
 
-#define AFID_FUSED_VARIANTS(X, V) \
-  X(V, uint32_t, true) X(V, uint32_t, false) X(V, uint64_t, true) X(V, uint64_t, false)
-
 hipError_t column_configure_lds(size_t bytes) {
   hipError_t e = hipSuccess;
-#define SETA1(V, O, R)                                                                                   \
-  e = hipFuncSetAttribute((const void *)k_fused_update<V, O, R>, hipFuncAttributeMaxDynamicSharedMemorySize, \
-                          (int)bytes);                                                                   \
-  if (e != hipSuccess) return e;
-#define SETA(V) AFID_FUSED_VARIANTS(SETA1, V)
-  AFID_LM_LIST(SETA)
-#undef SETA
-#undef SETA1
 #define SETB(V)                                                                                          \
   e = hipFuncSetAttribute((const void *)k_column_solve<V, uint32_t>, hipFuncAttributeMaxDynamicSharedMemorySize, \
                           (int)bytes);                                                                   \
@@ -1197,33 +940,6 @@ hipError_t column_configure_lds(size_t bytes) {
 static dim3 column_grid(const Geom &g) {
   int64_t nblk = (g.ncol + COLS_PER_WG - 1) / COLS_PER_WG;
   return dim3((unsigned)(((nblk + 7) / 8) * 8));
-}
-
-hipError_t launch_fused_update(const FusedArgs &a, hipStream_t st) {
-  const int64_t ntiles = (int64_t)((a.g.nym + 3) / 4) * ((a.g.nzm + 3) / 4);
-  const dim3 grid((unsigned)(((ntiles + 7) / 8) * 8));
-  // 32-bit byte offsets against the uniform base pointers while a field fits in 4 GiB
-  const bool small = (uint64_t)a.g.ncol * a.g.ldx * sizeof(double) < (1ull << 32);
-  const bool ru = a.sc.use_ru != 0;
-#define LAUNCH1(V, O, R)                                                                                    \
-  if (std::is_same<O, uint32_t>::value == small && R == ru) {                                              \
-    hipLaunchKernelGGL((k_fused_update<V, O, R>), grid, dim3(WG), column_lds(a.g), st, a, a.f.vx, a.f.vy, \
-                       a.f.vz, a.f.temp, a.f.pr, a.rux, a.ruy, a.ruz, a.rut, a.vxn, a.vyn, a.vzn, a.tn,  \
-                       a.stale, a.kc, a.fac);                                                              \
-    return hipGetLastError();                                                                              \
-  }
-  switch (pick_lm(a.g.L)) {
-#define CASE(V)                        \
-  case V: {                            \
-    AFID_FUSED_VARIANTS(LAUNCH1, V)    \
-    break;                             \
-  }
-    AFID_LM_LIST(CASE)
-#undef CASE
-#undef LAUNCH1
-    default: break;
-  }
-  return hipErrorInvalidValue;
 }
 
 hipError_t launch_stencil_rhs(const FusedArgs &a, hipStream_t st) {
@@ -1259,10 +975,10 @@ hipError_t launch_column_solve(const FusedArgs &a, hipStream_t st) {
   case V:                                                                                              \
     if (small)                                                                                         \
       hipLaunchKernelGGL((k_column_solve<V, uint32_t>), grid, dim3(SOLVE_WG), shm, st, a, a.f.vz, a.bx, \
-                         a.by, a.bz, a.bt, a.vxn, a.vyn, a.vzn, a.tn, a.stale, a.fac);                  \
+                         a.by, a.bz, a.bt, a.tn, a.stale, a.fac);                                        \
     else                                                                                               \
       hipLaunchKernelGGL((k_column_solve<V, uint64_t>), grid, dim3(SOLVE_WG), shm, st, a, a.f.vz, a.bx, \
-                         a.by, a.bz, a.bt, a.vxn, a.vyn, a.vzn, a.tn, a.stale, a.fac);                  \
+                         a.by, a.bz, a.bt, a.tn, a.stale, a.fac);                                        \
     return hipGetLastError();
     AFID_LM_LIST(CASE)
 #undef CASE
@@ -1297,13 +1013,19 @@ hipError_t launch_correct(const CorrectArgs &a, int which, hipStream_t st) {
   const int zplanes = ZM_DEFAULT;
   const int64_t nt = (int64_t)((a.g.nzm + zplanes - 1) / zplanes) * ((a.g.nym + SY_COLS - 1) / SY_COLS) *
                      ((a.g.ldx + 63) / 64);
-  hipLaunchKernelGGL(k_correct, dim3((unsigned)(((nt + 7) / 8) * 8)), dim3(SY_WG),
-                     sizeof(double) * (CZ_ROWS * 64 + 2 * SY_WG), st, a, which, zplanes);
+  const size_t shm = sizeof(double) * (CZ_ROWS * 64 + 2 * 2 * SY_WG);
+  const dim3 grid((unsigned)(((nt + 7) / 8) * 8));
+  if (a.cflbits) {
+    if (which != 3 || a.ux == a.vx) return hipErrorInvalidValue;   // the fold needs u* apart from the fields
+    hipLaunchKernelGGL(k_correct<true>, grid, dim3(SY_WG), shm, st, a, which, zplanes);
+  } else {
+    hipLaunchKernelGGL(k_correct<false>, grid, dim3(SY_WG), shm, st, a, which, zplanes);
+  }
   return hipGetLastError();
 }
 
-// LDS per block: WG*L doubles (fused update), SC_N*64 + 2*SV_N*SY_WG (stencil pass),
-// SYS_N*RW_N*16*L (column solve); L <= 80 keeps all within gfx950's 160 KiB.
+// LDS per block: WG*L doubles (per-routine solve), SC_N*64 + 2*SV_N*SY_WG (stencil
+// pass), SYS_N*RW_N*16*L (column solve); L <= 80 keeps all within gfx950's 160 KiB.
 int max_supported_L() { return 80; }
 size_t column_lds_need(const Geom &g) { return std::max(stencil_lds(g), solve_lds(g)); }
 

@@ -4,7 +4,8 @@
 //      divergence of u*, CalcLocalDivergence.F90, on the fly),
 //   2. z complex FFT (forward, and later backward),
 //   3. per-(m,n) tridiagonal solve in x (the zgttrf/zgttrs loop, :117-141,
-//      including the 1/(nym*nzm) normalisation of :105),
+//      including the 1/(nym*nzm) normalisation of :105), each mode's line
+//      held in registers across a group of lanes,
 //   4. y complex-to-real FFT back into dph.
 // FFTs are in-LDS mixed-radix (4,2,3,5) decimation in time, unnormalised,
 // forward sign -1, with FFTW's c2r convention (imaginary parts of the DC and
@@ -330,77 +331,208 @@ __global__ void __launch_bounds__(256) k_zfft(PoissonArgs a, R sign) {
 }
 
 // ---------------------------------------------------------------- x solve
-// One thread per (m,n).  Row k of the normalised system (diagonal 1):
+// Row k of the (m,n) mode's system, normalised to a unit diagonal
+// (SolvePressureCorrection.F90:117-141):
 //   amph(k) x(k-1) + x(k) + apph(k) x(k+1) = dphc(k)/(nym nzm) * acphT_b(k),
 //   acphT_b = 1/(acphk(k) - ak2(m) - ak1(n)), amph = amphk*acphT_b, apph = apphk*acphT_b.
-// The (0,0) mode is singular (zero row sums); its last pivot is left out and
-// x(nxm) := 0 fixes the additive constant that the reference leaves to rounding.
-// The recurrences walk k in blocks of XB rows whose spectrum loads are issued
-// together before the block's rows are eliminated.
-constexpr int XB = 8;
-template <typename R>
-__global__ void __launch_bounds__(256) k_xsolve_pressure(PoissonArgs a) {
-  using C = cplx_t<R>;
-  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int nymh = a.nymh, nzm = a.g.nzm, nxm = a.g.nxm;
-  if (id >= (int64_t)nymh * nzm || (a.ds && a.ds->halted)) return;
-  const int n = (int)(id / nymh), m = (int)(id - (int64_t)n * nymh);
-  const int64_t stride = (int64_t)nzm * a.ldm;
-  C *S = reinterpret_cast<C *>(a.S) + (int64_t)n * a.ldm + m;
-  double *cp = a.cp + (int64_t)n * a.ldm + m;
-  const double lam = a.ak2[m] + 0.0;  // ak2 indexed by y wavenumber
-  const double ak1n = a.ak1[n];
-  const double nrm = (double)(nzm * a.g.nym);
-  const bool zero_mode = (m == 0 && n == 0);
-  double u_prev = 1.0, du_prev = 0.0;
-  double2 y_prev = make_double2(0.0, 0.0);
-  for (int k0 = 0; k0 < nxm; k0 += XB) {
-    double2 rb[XB];
+// It is eliminated in Thomas order without pivoting: u(0) = 1,
+// l(k) = amph(k)/u(k-1), u(k) = 1 - l(k) apph(k-1), y(k) = r(k) - l(k) y(k-1),
+// z(k) = y(k)/u(k), x(k) = z(k) - (apph(k)/u(k)) x(k+1).  The (0,0) mode is
+// singular (zero row sums); its last pivot is left out and x(nxm) := 0 fixes
+// the additive constant that the reference leaves to rounding.
+//
+// The whole x line of a mode stays in registers: NP lanes own chunks of
+// Lc = ceil(nxm/NP) consecutive rows each, so the spectrum is read once and
+// written once.  The pivot recurrence u(k) is nonlinear, so it is not scanned:
+// the pivot at the row before each chunk, u(k0-1), comes from a table filled
+// once per grid by k_xsolve_pivots (the coefficients do not depend on dt), and
+// each lane continues the same recurrence through its own rows.  The two
+// affine recurrences (y forward, x backward) are run per chunk from a zero
+// carry and joined by a scan of the chunk maps over the NP lanes; each
+// lane's rows then take the true carry in a short fix-up pass.
+template <int W>
+__device__ __forceinline__ double2 scan_up_c(double A, double2 B, int p) {
 #pragma unroll
-    for (int t = 0; t < XB; t++) rb[t] = widen(S[min(k0 + t, nxm - 1) * stride]);
-#pragma unroll
-    for (int t = 0; t < XB; t++) {
-      const int k = k0 + t;
-      if (k >= nxm) break;
-      const double b = 1.0 / (a.acphk[k] - lam - ak1n);
-      const double dl = a.amphk[k] * b, du = a.apphk[k] * b;
-      double2 r = rb[t];
-      r.x = r.x / nrm * b;
-      r.y = r.y / nrm * b;
-      double u = 1.0;
-      if (k > 0) {
-        const double l = dl / u_prev;
-        u = 1.0 - l * du_prev;
-        r.x = r.x - l * y_prev.x;
-        r.y = r.y - l * y_prev.y;
-      }
-      y_prev = r;
-      u_prev = u;
-      du_prev = du;
-      double ui = (zero_mode && k == nxm - 1) ? 0.0 : 1.0 / u;
-      S[k * stride] = narrow<C>(make_double2(r.x * ui, r.y * ui));
-      cp[k * stride] = du * ui;
+  for (int off = 1; off < W; off <<= 1) {
+    const double A2 = __shfl_up(A, off, W), Bx = __shfl_up(B.x, off, W), By = __shfl_up(B.y, off, W);
+    if (p >= off) {
+      B.x = A * Bx + B.x;
+      B.y = A * By + B.y;
+      A = A * A2;
     }
   }
-  double2 x = widen(S[(nxm - 1) * stride]);
-  for (int k1 = nxm - 2; k1 >= 0; k1 -= XB) {
-    double2 yb[XB];
-    double cb[XB];
+  const double cx = __shfl_up(B.x, 1, W), cy = __shfl_up(B.y, 1, W);
+  return p == 0 ? make_double2(0.0, 0.0) : make_double2(cx, cy);
+}
+template <int W>
+__device__ __forceinline__ double2 scan_down_c(double A, double2 B, int p) {
 #pragma unroll
-    for (int t = 0; t < XB; t++) {
-      const int k = max(k1 - t, 0);
-      yb[t] = widen(S[k * stride]);
-      cb[t] = cp[k * stride];
+  for (int off = 1; off < W; off <<= 1) {
+    const double A2 = __shfl_down(A, off, W), Bx = __shfl_down(B.x, off, W), By = __shfl_down(B.y, off, W);
+    if (p + off < W) {
+      B.x = A * Bx + B.x;
+      B.y = A * By + B.y;
+      A = A * A2;
     }
+  }
+  const double cx = __shfl_down(B.x, 1, W), cy = __shfl_down(B.y, 1, W);
+  return p == W - 1 ? make_double2(0.0, 0.0) : make_double2(cx, cy);
+}
+
+// Pivot table: piv[mode*np + p] = u(p*lc - 1) for the chunks p >= 1 that hold
+// rows (mode = n*nymh + m); one thread per mode walks the recurrence.
+__global__ void __launch_bounds__(256) k_xsolve_pivots(PoissonArgs a, double *piv, int np, int lc) {
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int nymh = a.nymh, nzm = a.g.nzm, nxm = a.g.nxm;
+  if (id >= (int64_t)nymh * nzm) return;
+  const int n = (int)(id / nymh), m = (int)(id - (int64_t)n * nymh);
+  const double lam = a.ak2[m] + 0.0, ak1n = a.ak1[n];
+  double *row = piv + id * np;
+  row[0] = 1.0;
+  double u_prev = 1.0, du_prev = 0.0;
+  for (int k = 0; k < nxm; k++) {
+    const double b = 1.0 / (a.acphk[k] - lam - ak1n);
+    const double dl = a.amphk[k] * b, du = a.apphk[k] * b;
+    double u = 1.0;
+    if (k > 0) {
+      const double l = dl / u_prev;
+      u = 1.0 - l * du_prev;
+    }
+    const int q = (k + 1) / lc;
+    if ((k + 1) - q * lc == 0 && q < np) row[q] = u;
+    u_prev = u;
+    du_prev = du;
+  }
+}
+
+template <int NP, int LM, typename R>
+__global__ void __launch_bounds__(256) k_xsolve_pressure(PoissonArgs a) {
+  using C = cplx_t<R>;
+  constexpr int MPB = 256 / NP;   // modes per block, consecutive in m
+  extern __shared__ double xco[];   // [3][nxm]: acphk, amphk, apphk
+  const int nymh = a.nymh, nzm = a.g.nzm, nxm = a.g.nxm;
+  const int nmb = (nymh + MPB - 1) / MPB;
+  const int64_t tile = xcd_block((int64_t)nmb * nzm);
+  if (tile < 0 || (a.ds && a.ds->halted)) return;
+  for (int i = threadIdx.x; i < nxm; i += 256) {
+    xco[i] = a.acphk[i];
+    xco[nxm + i] = a.amphk[i];
+    xco[2 * nxm + i] = a.apphk[i];
+  }
+  const int n = (int)(tile / nmb), mb = (int)(tile - (int64_t)n * nmb);
+  const int p = (int)threadIdx.x % NP;
+  const int m = mb * MPB + (int)threadIdx.x / NP;
+  const bool live = m < nymh;
+  const int mc = live ? m : nymh - 1;   // lanes past the last mode solve a valid one and store nothing
+  const int lc = (nxm + NP - 1) / NP;
+  const int k0 = p * lc;
+  const int cnt = max(0, min(lc, nxm - k0));
+  const int64_t stride = (int64_t)nzm * a.ldm;
+  C *S = reinterpret_cast<C *>(a.S) + (int64_t)n * a.ldm + mc;
+  double2 y[LM];
 #pragma unroll
-    for (int t = 0; t < XB; t++) {
-      const int k = k1 - t;
-      if (k < 0) break;
-      x = make_double2(yb[t].x - cb[t] * x.x, yb[t].y - cb[t] * x.y);
-      S[k * stride] = narrow<C>(x);
+  for (int t = 0; t < LM; t++) y[t] = widen(S[(int64_t)min(k0 + min(t, lc - 1), nxm - 1) * stride]);
+  const double lam = a.ak2[mc] + 0.0, ak1n = a.ak1[n];
+  const double nrm = (double)(nzm * a.g.nym);
+  const bool zero_mode = (mc == 0 && n == 0);
+  double u0 = 1.0, du0 = 0.0;   // u and apph of row k0-1
+  if (p > 0 && cnt > 0) {
+    u0 = a.piv[((int64_t)n * nymh + mc) * NP + p];
+    const double bp = 1.0 / (a.acphk[k0 - 1] - lam - ak1n);
+    du0 = a.apphk[k0 - 1] * bp;
+  }
+  __syncthreads();
+  const double *ac = xco, *am = xco + nxm, *ap = xco + 2 * nxm;
+  // pass 1: pivots u(k) of the chunk (kept in w) and the chunk's y from a zero carry
+  double w[LM];
+  double Ay = 1.0;
+  double2 yl = make_double2(0.0, 0.0);
+  {
+    double u_prev = u0, du_prev = du0;
+#pragma unroll
+    for (int t = 0; t < LM; t++) {
+      w[t] = 1.0;
+      if (t < cnt) {
+        const int k = k0 + t;
+        const double b = 1.0 / (ac[k] - lam - ak1n);
+        const double dl = am[k] * b, du = ap[k] * b;
+        double2 r = y[t];
+        r.x = r.x / nrm * b;
+        r.y = r.y / nrm * b;
+        double u = 1.0;
+        if (k > 0) {
+          const double l = dl / u_prev;
+          u = 1.0 - l * du_prev;
+          r.x = r.x - l * yl.x;
+          r.y = r.y - l * yl.y;
+          Ay *= -l;
+        }
+        yl = r;
+        y[t] = r;
+        w[t] = u;
+        u_prev = u;
+        du_prev = du;
+      }
+    }
+  }
+  // pass 2: y(k0-1) from the lanes before; z = y/u; w becomes apph/u
+  {
+    double2 d = scan_up_c<NP>(Ay, yl, p);
+    double u_prev = u0, du_prev = du0;
+#pragma unroll
+    for (int t = 0; t < LM; t++) {
+      if (t < cnt) {
+        const int k = k0 + t;
+        const double b = 1.0 / (ac[k] - lam - ak1n);
+        const double dl = am[k] * b, du = ap[k] * b;
+        const double u = w[t];
+        if (k > 0) {
+          const double l = dl / u_prev;
+          d.x = -l * d.x;
+          d.y = -l * d.y;
+        }
+        const double uinv = (zero_mode && k == nxm - 1) ? 0.0 : 1.0 / u;
+        y[t].x = (y[t].x + d.x) * uinv;
+        y[t].y = (y[t].y + d.y) * uinv;
+        w[t] = du * uinv;
+        u_prev = u;
+        du_prev = du;
+      }
+    }
+  }
+  // pass 3: back substitution of the chunk from a zero carry
+  double Ax = 1.0;
+  double2 xl = make_double2(0.0, 0.0);
+#pragma unroll
+  for (int t = LM - 1; t >= 0; t--) {
+    if (t < cnt) {
+      xl.x = y[t].x - w[t] * xl.x;
+      xl.y = y[t].y - w[t] * xl.y;
+      y[t] = xl;
+      Ax *= -w[t];
+    }
+  }
+  // pass 4: x(k0+cnt) from the lanes after, and the stores
+  double2 e = scan_down_c<NP>(Ax, xl, p);
+#pragma unroll
+  for (int t = LM - 1; t >= 0; t--) {
+    if (t < cnt) {
+      e.x = -w[t] * e.x;
+      e.y = -w[t] * e.y;
+      if (live) S[(int64_t)(k0 + t) * stride] = narrow<C>(make_double2(y[t].x + e.x, y[t].y + e.y));
     }
   }
 }
+
+// Lanes per mode and register rows per lane of the x solve for nxm rows.
+void xsolve_shape(int nxm, int *np, int *lm) {
+  if (nxm <= 32 * 4) { *np = 32; *lm = 4; }
+  else if (nxm <= 32 * 8) { *np = 32; *lm = 8; }
+  else if (nxm <= 32 * 12) { *np = 32; *lm = 12; }
+  else if (nxm <= 64 * 12) { *np = 64; *lm = 12; }
+  else { *np = 64; *lm = 20; }
+}
+int xsolve_max_rows() { return 64 * 20; }
 
 // ---------------------------------------------------------------- y inverse
 template <typename R>
@@ -515,11 +647,30 @@ hipError_t launch_zfft(const PoissonArgs &a, int sign, hipStream_t st) {
   else hipLaunchKernelGGL(k_zfft<double>, grid, dim3(256), shm, st, a, (double)sign);
   return hipGetLastError();
 }
+hipError_t launch_xsolve_pivots(const PoissonArgs &a, double *piv, hipStream_t st) {
+  int np, lm;
+  xsolve_shape(a.g.nxm, &np, &lm);
+  const int64_t n = (int64_t)a.nymh * a.g.nzm;
+  hipLaunchKernelGGL(k_xsolve_pivots, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, piv, np,
+                     (a.g.nxm + np - 1) / np);
+  return hipGetLastError();
+}
+template <int NP, int LM>
+static void launch_xsolve_r(const PoissonArgs &a, hipStream_t st) {
+  const int64_t nmb = (a.nymh + 256 / NP - 1) / (256 / NP);
+  const dim3 grid((unsigned)(((nmb * a.g.nzm + 7) / 8) * 8));
+  const size_t shm = sizeof(double) * 3 * a.g.nxm;
+  if (a.spec32) hipLaunchKernelGGL((k_xsolve_pressure<NP, LM, float>), grid, dim3(256), shm, st, a);
+  else hipLaunchKernelGGL((k_xsolve_pressure<NP, LM, double>), grid, dim3(256), shm, st, a);
+}
 hipError_t launch_xsolve_pressure(const PoissonArgs &a, hipStream_t st) {
-  int64_t n = (int64_t)a.nymh * a.g.nzm;
-  dim3 grid((unsigned)((n + 255) / 256));
-  if (a.spec32) hipLaunchKernelGGL(k_xsolve_pressure<float>, grid, dim3(256), 0, st, a);
-  else hipLaunchKernelGGL(k_xsolve_pressure<double>, grid, dim3(256), 0, st, a);
+  int np, lm;
+  xsolve_shape(a.g.nxm, &np, &lm);
+  if (np == 32 && lm == 4) launch_xsolve_r<32, 4>(a, st);
+  else if (np == 32 && lm == 8) launch_xsolve_r<32, 8>(a, st);
+  else if (np == 32) launch_xsolve_r<32, 12>(a, st);
+  else if (lm == 12) launch_xsolve_r<64, 12>(a, st);
+  else launch_xsolve_r<64, 20>(a, st);
   return hipGetLastError();
 }
 hipError_t launch_yfft_inverse(const PoissonArgs &a, hipStream_t st) {

@@ -7,12 +7,12 @@ namespace afid {
 struct FusedArgs {
   Geom g;
   FieldPtrs f;                       // fields at the start of the substep
-  double *vxn, *vyn, *vzn, *tn;      // updated fields (separate buffers)
+  double *tn;                        // updated temperature
   double *rux, *ruy, *ruz, *rut;     // explicit-term history, updated in place
   double *stale;                     // rhs(1,j,i) of the reference, nym*nzm
   const double *kc;                  // KC_N interleaved coefficient rows
   const double *fac;                 // SYS_N*FR_N interleaved factor rows
-  double *bx, *by, *bz, *bt;         // right-hand sides b + A q_old of the split update
+  double *bx, *by, *bz, *bt;         // right-hand sides b + A q_old; then u* (bx, by, bz)
   StepScalars sc;
   const DevStep *ds;                 // non-null: scalars of substep ns come from the device
   int ns;
@@ -44,11 +44,13 @@ struct SolveArgs {
 struct CorrectArgs {
   Geom g;
   const double *dph;
+  const double *ux, *uy, *uz;   // velocities before the correction (may equal vx, vy, vz)
   double *vx, *vy, *vz, *pr;
   const double *kc;
   StepScalars sc;
   const DevStep *ds;
   int ns;
+  unsigned long long *cflbits;  // non-null: CalcMaxCFL of the corrected velocities, atomic max of the bits
 };
 
 // One complex FFT length with its radix stages, twiddles exp(-2 pi i t/n)
@@ -75,7 +77,7 @@ struct PoissonArgs {
   Geom g;
   int nymh, ldm;         // y wavenumbers kept (nym/2+1), padded row length
   double2 *S;            // spectral array S[k][n][m]
-  double *cp;            // x-solve scratch, same indexing as S
+  const double *piv;     // x-solve pivots at chunk starts, filled once per grid (launch_xsolve_pivots)
   const double *kc;      // for KC_ACPHK/AMPHK/APPHK (read through kx arrays below)
   const double *amphk, *acphk, *apphk;  // natural k order, length nxm
   const double *ak1, *ak2;              // z (nzm) and y (nym) modified wavenumbers
@@ -91,7 +93,6 @@ struct PoissonArgs {
   int spec32;            // S holds FP32 complex values and the y/z FFTs run in FP32 (mixed precision)
 };
 
-hipError_t launch_fused_update(const FusedArgs &a, hipStream_t st);
 hipError_t launch_stencil_rhs(const FusedArgs &a, hipStream_t st);
 hipError_t launch_column_solve(const FusedArgs &a, hipStream_t st);
 hipError_t launch_solve_update(const SolveArgs &a, int which, hipStream_t st);
@@ -106,6 +107,9 @@ hipError_t column_configure_lds(size_t bytes);
 hipError_t launch_yfft_forward(const PoissonArgs &a, hipStream_t st);
 hipError_t launch_zfft(const PoissonArgs &a, int sign, hipStream_t st);
 hipError_t launch_xsolve_pressure(const PoissonArgs &a, hipStream_t st);
+hipError_t launch_xsolve_pivots(const PoissonArgs &a, double *piv, hipStream_t st);
+void xsolve_shape(int nxm, int *np, int *lm);   // lanes per mode, register rows per lane
+int xsolve_max_rows();
 hipError_t launch_yfft_inverse(const PoissonArgs &a, hipStream_t st);
 int fft_max_len();
 hipError_t fft_configure_lds(size_t bytes);
@@ -169,6 +173,8 @@ struct FactorArgs {
   DevStep *ds;
   StepConsts k;
 };
+// cfl_out: CalcMaxCFL from k_red_cfl; null: from ds->cflbits, which the last
+// correction of the previous step (or the host, before the first) filled.
 hipError_t launch_step_begin(DevStep *ds, const double *cfl_out, StepConsts k, hipStream_t st);
 hipError_t launch_factor(const FactorArgs &a, hipStream_t st);
 hipError_t launch_step_end(DevStep *ds, hipStream_t st);

@@ -1,7 +1,8 @@
 // step_kernels.hip -- time-step control on the device (DevStep), so that whole
 // steps of main.F90 run without a host round trip and can be replayed as one
 // captured graph:
-//   k_step_begin  CalcMaxCFL result -> dt and error code (main.F90:169-189),
+//   k_step_begin  CalcMaxCFL result (from k_red_cfl, or folded into the last
+//                 correction of the previous step) -> dt and error code (main.F90:169-189),
 //                 substep scalars al, ga, ro, beta (TimeMarcher.F90:22-30)
 //   k_factor      the wall-normal systems of each substep for that dt
 //                 (SolveImpEqnUpdate_X.F90:27-44, _YZ.F90:25-38, _Temp.F90:31-47)
@@ -15,7 +16,9 @@ namespace afid {
 
 __global__ void k_step_begin(DevStep *ds, const double *cfl_out, StepConsts k) {
   if (threadIdx.x != 0 || ds->halted) return;
-  const double instcfl = fmax(0.00000001, cfl_out[0]);   // CalcMaxCFL floor
+  const double cfl = cfl_out ? cfl_out[0] : __longlong_as_double((long long)ds->cflbits);
+  ds->cflbits = 0;   // the last correction of this step accumulates the next step's value
+  const double instcfl = fmax(0.00000001, cfl);   // CalcMaxCFL floor
   double dt = ds->dtfix;
   int err = 0;
   if (ds->idtv) {

@@ -97,6 +97,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "afid_steps_more": ([vp, i], i),
         "afid_steps_query": ([vp, ctypes.POINTER(i)], i),
         "afid_steps_wait": ([vp, pd, ctypes.POINTER(i), ctypes.POINTER(i)], i),
+        "afid_steps_cfl": ([vp, pd], i),
         "afid_batch_steps": ([ctypes.POINTER(vp), i, i], i),
         "afid_estimate_bytes": ([ctypes.c_void_p, ctypes.POINTER(i64)], i),
         "afid_tune": ([vp, ctypes.POINTER(i)], i),
@@ -409,9 +410,16 @@ class Cell:
         self._check(self.L.afid_steps_async(self.h, ntime0, nsteps, idtv, dt, dtmin, dtmax, limitcfl),
                     "steps_async")
 
+    def steps_cfl(self) -> float:
+        """CalcMaxCFL of the fields after the last device step, as the device step
+        control computed it (folded into the last correction in FP64)."""
+        v = ctypes.c_double()
+        self._check(self.L.afid_steps_cfl(self.h, ctypes.byref(v)), "steps_cfl")
+        return v.value
+
     def tune(self) -> int:
         """Autotunes the stencil pass's z planes per block on this cell (call before
-        setting the initial state); returns the choice (0: fused update, untuned)."""
+        setting the initial state); returns the chosen number of planes."""
         zm = ctypes.c_int()
         self._check(self.L.afid_tune(self.h, ctypes.byref(zm)), "tune")
         return zm.value

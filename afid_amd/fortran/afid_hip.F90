@@ -333,6 +333,12 @@
           real(c_double), intent(out) :: dt_last
           integer(c_int), intent(out) :: errorcode, steps_done
         end function
+        integer(c_int) function afid_steps_cfl(ctx, cflm)                &
+     &      bind(C, name='afid_steps_cfl')
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: ctx
+          real(c_double), intent(out) :: cflm
+        end function
         integer(c_int) function afid_batch_steps(cells, ncells, nsteps) &
      &      bind(C, name='afid_batch_steps')
           import :: c_int, c_ptr

@@ -59,9 +59,8 @@ typedef struct afid_params {
  * wall-normal factorisations and solves, the pressure x-solve arithmetic and
  * all reductions stay FP64, and each step ends with one refinement sweep: the
  * divergence left after the last correction, computed in FP64, is solved for
- * and corrected once more.  Needs the default split update (not
- * AFID_UPDATE=fused); the per-routine ImplicitAndUpdate entry points return
- * AFID_ERR_STATE in this mode. */
+ * and corrected once more.  The per-routine ImplicitAndUpdate entry points
+ * return AFID_ERR_STATE in this mode. */
 #define AFID_PREC_FP64 0
 #define AFID_PREC_MIXED 1
 
@@ -163,13 +162,20 @@ int afid_step(afid_ctx *ctx, int ntime, int idtv, double dt, double dtmin, doubl
  * interchange. */
 int afid_steps(afid_ctx *ctx, int ntime0, int nsteps, int idtv, double dt, double dtmin, double dtmax,
                double limitcfl, double *dt_last, int *errorcode, int *steps_done);
+/* CalcMaxCFL (CalcMaxCFL.F90:11-48, with its 1e-8 floor) of the fields the
+ * last device step left, as the device step control computed it for the next
+ * step: in FP64 the step graph folds the reduction into its last pressure
+ * correction, so the fields are not read again.  Equals afid_calc_max_cfl of
+ * the same fields.  AFID_ERR_STATE before any device step and in the mixed
+ * mode, whose step graph reduces at the start of each step instead. */
+int afid_steps_cfl(afid_ctx *ctx, double *cflm);
 
 /* ---- concurrent cells (batched sweeps, serving) --------------------------------
  * Each context has its own stream and its own captured step graph, so the
  * steps of several cells overlap on the GPU.  No counterpart in the
  * reference, which runs one cell per job. */
 /* afid_steps without the final wait: sets the device step state as afid_steps
- * does, queues nsteps steps and returns.  AFID_ERR_STATE with AFID_UPDATE=fused. */
+ * does, queues nsteps steps and returns. */
 int afid_steps_async(afid_ctx *ctx, int ntime0, int nsteps, int idtv, double dt, double dtmin, double dtmax,
                      double limitcfl);
 /* Queues nsteps more steps that continue the device step state (time, ntime,
@@ -183,9 +189,9 @@ int afid_steps_wait(afid_ctx *ctx, double *dt_last, int *errorcode, int *steps_d
 /* Queues nsteps more steps on each of ncells contexts, step n of every cell
  * before step n+1 of any; each cell must have had afid_steps_async. */
 int afid_batch_steps(afid_ctx *const *cells, int ncells, int nsteps);
-/* Autotuner for the stencil pass (split update): times z planes per block in
+/* Autotuner for the stencil pass: times z planes per block in
  * {8, 16, 32, 64} (not above nzm) on this context and keeps the fastest for
- * its later launches; *zm receives the choice (0 with AFID_UPDATE=fused).  It
+ * its later launches; *zm receives the choice.  It
  * writes the histories and scratch while timing and clears them afterwards:
  * call it before setting the initial state.  No counterpart in the
  * reference. */

@@ -187,14 +187,12 @@ def test_ab2_scheme(afid):
     cell.close()
 
 
-@pytest.mark.parametrize("path", ["split", "fused"])
 @pytest.mark.parametrize("nsst", [3, 1])
-def test_histories_kept_where_read(afid, nsst, path, monkeypatch):
+def test_histories_kept_where_read(afid, nsst):
     """Each substep stores its explicit terms in rux..rutemp as the reference
-    does (both update paths), except RK3 substep 3: the next step's substep 1 has ro = 0 and never
+    does, except RK3 substep 3: the next step's substep 1 has ro = 0 and never
     reads them, so that pass leaves substep 2's terms in place.  The step after
     it still matches the oracle, which stores them."""
-    monkeypatch.setenv("AFID_UPDATE", path)
     c = dict(SMALL_CASES[0])
     o, cell = make_pair(afid, c, nsst=nsst)
     o.init_linear_noise(4, 1e-3)
@@ -265,25 +263,23 @@ def test_conduction_fixed_point_on_gpu(afid):
 
 
 @pytest.mark.parametrize("c", SMALL_CASES, ids=case_id)
-def test_fused_and_split_update_agree(afid, c, monkeypatch):
-    """The one-pass update (AFID_UPDATE=fused) and the default stencil + column
-    solve pair compute the same substep; both are checked against the oracle."""
-    o, _ = make_pair(afid, c)
-    o.init_linear_noise(5, 1e-3)
-    out = {}
-    for path in ("split", "fused"):
-        monkeypatch.setenv("AFID_UPDATE", path)
-        cell = afid.Cell(afid.CellParams(nsst=3, alx3=1.0, **c))
-        cell.init_linear_noise(5, 1e-3)
-        for n in range(4):
-            cell.step(n, 1, 2e-3, 1e-4, 5e-3, 1.2)
-        out[path] = {nm: cell.download(nm) for nm in ("vx", "vy", "vz", "temp", "pr")}
-        cell.close()
-    for n in range(4):
-        o.step(n, 1, 2e-3, 1e-4, 5e-3, 1.2)
-    for nm in ("vx", "vy", "vz", "temp"):
-        assert rel_l2(out["split"][nm], out["fused"][nm]) < 1e-13, nm
-        assert rel_l2(out["split"][nm], o.interior(nm)) < 1e-11, nm
+def test_folded_cfl_equals_reduction(afid, c):
+    """The FP64 step graph takes CalcMaxCFL from its last pressure correction
+    instead of a separate pass over the fields: after every step the value it
+    carries equals CalcMaxCFL of the stored fields bit for bit, so the dt
+    sequence is the one a separate reduction gives.  Also for an upload
+    between calls, after which the next call starts from a fresh reduction."""
+    cell = afid.Cell(afid.CellParams(nsst=3, alx3=1.0, **c))
+    cell.init_linear_noise(6, 1e-3)
+    for n in range(5):
+        err, _, done = cell.steps(n, 1 if n < 2 else 2, 1, 2e-3, 1e-4, 5e-3, 1.2)
+        assert (err, done) == (0, 1 if n < 2 else 2)
+        assert cell.steps_cfl() == cell.CalcMaxCFL(), n
+    vx = cell.download("vx")
+    cell.upload("vx", 2.0 * vx)
+    cell.steps(20, 1, 1, 2e-3, 1e-4, 5e-3, 1.2)
+    assert cell.steps_cfl() == cell.CalcMaxCFL()
+    cell.close()
 
 
 @pytest.mark.parametrize("c", SMALL_CASES, ids=case_id)

@@ -10,10 +10,9 @@ below 1e-9 for temp (1.9e-12) and below 1e-6 for pr modulo its mean (2.6e-9),
 and CheckDivergence stays below 1e-8.  The FP64 path is itself checked
 against the oracle elsewhere, so these bounds are the error against the oracle
 up to that path's 1e-11.  History fields round-trip through upload/download
-as FP32 values; the per-routine ImplicitAndUpdate entry points and the fused
-update are refused in this mode.
+as FP32 values; the per-routine ImplicitAndUpdate entry points are refused in
+this mode.
 """
-import os
 
 import numpy as np
 import pytest
@@ -78,13 +77,9 @@ def test_mixed_refusals():
     c.ExplicitTermsVX()
     with pytest.raises(AfidError, match="code -5"):
         c.ImplicitAndUpdateVX()
+    with pytest.raises(AfidError, match="code -5"):
+        c.steps_cfl()
     c.close()
-    os.environ["AFID_UPDATE"] = "fused"
-    try:
-        with pytest.raises(AfidError, match="code -1"):
-            _cell(PREC_MIXED)
-    finally:
-        del os.environ["AFID_UPDATE"]
 
 
 def test_params_precision_field():
