@@ -101,6 +101,31 @@ struct DevStep {
   unsigned long long cflbits;   // CalcMaxCFL for the next step (bits of a non-negative double)
 };
 
+// Values that one kernel of a step writes and a later kernel reads (the
+// device step state, the stale row, reduction results read by one thread)
+// are loaded with device-coherent vector loads (agent-scope atomic loads from
+// L2).  A plain load at a uniform address compiles to a scalar load, which a
+// later kernel of a replayed step graph can find served from the scalar cache
+// with an earlier step's line.
+__device__ __forceinline__ int coh_int(const int *p) {
+  return __hip_atomic_load(const_cast<int *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long coh_u64(const unsigned long long *p) {
+  return __hip_atomic_load(const_cast<unsigned long long *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double coh_dbl(const double *p) {
+  return __longlong_as_double((long long)coh_u64(reinterpret_cast<const unsigned long long *>(p)));
+}
+__device__ __forceinline__ StepScalars coh_scalars(const StepScalars *p) {
+  static_assert(sizeof(StepScalars) % 8 == 0, "StepScalars is read in 8-byte words");
+  StepScalars s;
+  const unsigned long long *src = reinterpret_cast<const unsigned long long *>(p);
+  unsigned long long *dst = reinterpret_cast<unsigned long long *>(&s);
+#pragma unroll
+  for (int q = 0; q < (int)(sizeof(StepScalars) / 8); q++) dst[q] = coh_u64(src + q);
+  return s;
+}
+
 // Host-set constants the device step control needs to rebuild StepScalars.
 struct StepConsts {
   double ren, pec, dy, dz, dyq, dzq, tempbp, temptp;

@@ -250,12 +250,13 @@ __device__ __forceinline__ void sth(double *base, OffT boff, double v) {
 // substep 3, since the next step's substep 1 has ro = 0) are not stored.
 // k_column_solve: a wave owns four columns (16 chunk lanes each) and solves the
 // four systems in registers: A q_new = b', with the factor rows staged in LDS.
-// The new velocities u* (before the pressure correction) overwrite b' in the
-// b buffers, each lane rewriting only the positions it read; the divergence
-// and the correction read u* there, and the correction writes the fields, so
-// the fields keep the substep's starting values until then.  The new
-// temperature is final and goes to its field.  The vz increment at k=1,
-// q_new - q_old, becomes the new stale row and the temperature's rhs(1).
+// The new velocities u* (before the pressure correction) go to (ux, uy, uz):
+// the fields themselves, in place (each lane rewrites only positions of its
+// own column, and reads vz at k = 1 before writing it), or, where the
+// correction must see u* apart from the corrected fields, the b buffers over
+// b' itself.  The new temperature is final and goes to its field.  The vz
+// increment at k=1, q_new - q_old, becomes the new stale row and the
+// temperature's rhs(1).
 constexpr int ZM_DEFAULT = 32;   // z planes per block unless afid_tune chose another count
 constexpr int SY_COLS = 4;       // y columns per block, one wave each
 constexpr int SY_WG = SY_COLS * 64;
@@ -318,8 +319,8 @@ __global__ void __launch_bounds__(SY_WG)
   const int q = (int)(tile % nq);
   const int64_t rest = tile / nq;
   const int gr = (int)(rest % ngr), ib = (int)(rest / ngr);
-  if (DEV && a.ds->halted) return;
-  const StepScalars c = DEV ? a.ds->sc[a.ns - 1] : a.sc;
+  if (DEV && coh_int(&a.ds->halted)) return;
+  const StepScalars c = DEV ? coh_scalars(&a.ds->sc[a.ns - 1]) : a.sc;
   const bool keep_ru = a.keep_ru != 0;
   for (int idx = threadIdx.x; idx < SC_N * 64; idx += SY_WG) {
     const int r = idx / 64, sl = min(q * 64 + idx - r * 64, ldx - 1);
@@ -375,7 +376,7 @@ __global__ void __launch_bounds__(SY_WG)
     }
     const double rxo = USE_RU ? ldh<H32>(rux, C + sb) : 0.0, ryo = USE_RU ? ldh<H32>(ruy, C + sb) : 0.0;
     const double rzo = USE_RU ? ldh<H32>(ruz, C + sb) : 0.0, rto = USE_RU ? ldh<H32>(rut, C + sb) : 0.0;
-    const double st0 = stalep[(int64_t)ic * g.nym + jc];
+    const double st0 = coh_dbl(stalep + (int64_t)ic * g.nym + jc);
     const double stale0 = k == 0 ? st0 : 0.0;
     double *buf = xv + par * (SV_N * SY_WG);
     buf[SV_VX * SY_WG + tid] = cur.vxc;
@@ -531,13 +532,13 @@ static size_t solve_lds(const Geom &g) { return sizeof(double) * SYS_N * RW_N * 
 
 template <int LM, typename OffT>
 __global__ void __launch_bounds__(SOLVE_WG)
-    k_column_solve(FusedArgs a, const double *__restrict__ vz, double *__restrict__ bx, double *__restrict__ by,
-                   double *__restrict__ bz, const double *__restrict__ bt, double *__restrict__ tn,
+    k_column_solve(FusedArgs a, const double *vz, const double *bx, const double *by, const double *bz,
+                   const double *__restrict__ bt, double *ox, double *oy, double *oz, double *__restrict__ tn,
                    double *__restrict__ stalep, const double *__restrict__ facp) {
   extern __shared__ double rows[];   // [SYS_N][RW_N][ldx]
   const Geom g = a.g;
   const int64_t blk = xcd_block((g.ncol + SOLVE_COLS - 1) / SOLVE_COLS);
-  if (blk < 0 || (a.ds && a.ds->halted)) return;
+  if (blk < 0 || (a.ds && coh_int(&a.ds->halted))) return;
   const int L = g.L, ldx = g.ldx, nx = g.nx, nxm = g.nxm;
   // the factor rows are the same for every column: stage them once per block
   for (int idx = threadIdx.x; idx < SYS_N * RW_N * ldx; idx += SOLVE_WG) {
@@ -566,7 +567,7 @@ __global__ void __launch_bounds__(SOLVE_WG)
       for (int t = 0; t < LM; t++)
         if (t < L) {   // slots of k >= nx (t >= cntx) are written 0
           const int k = k0 + t, s = t * P + p;
-          stf(bx, C + (OffT)s * sizeof(double), (k >= 1 && k <= nx - 2) ? b[t] + fh[s] * cin : 0.0);
+          stf(ox, C + (OffT)s * sizeof(double), (k >= 1 && k <= nx - 2) ? b[t] + fh[s] * cin : 0.0);
         }
   }
   load_chunk<LM, OffT>(b, by, C, cntm, L, p);
@@ -579,7 +580,7 @@ __global__ void __launch_bounds__(SOLVE_WG)
         if (t < L) {
           const int k = k0 + t, s = t * P + p;
           // row nx is outside the nxm system: nothing reads it from here
-          if (k != nx - 1) stf(by, C + (OffT)s * sizeof(double), k < nxm ? b[t] + fh[s] * cin : 0.0);
+          if (k != nx - 1) stf(oy, C + (OffT)s * sizeof(double), k < nxm ? b[t] + fh[s] * cin : 0.0);
         }
   }
   load_chunk<LM, OffT>(b, bz, C, cntm, L, p);
@@ -594,7 +595,7 @@ __global__ void __launch_bounds__(SOLVE_WG)
       for (int t = 0; t < LM; t++)
         if (t < L) {
           const int k = k0 + t, s = t * P + p;
-          if (k != nx - 1) stf(bz, C + (OffT)s * sizeof(double), k < nxm ? b[t] + fh[s] * cin : 0.0);
+          if (k != nx - 1) stf(oz, C + (OffT)s * sizeof(double), k < nxm ? b[t] + fh[s] * cin : 0.0);
         }
   }
   dvz0 = __shfl(dvz0, 0, P);
@@ -617,6 +618,367 @@ __global__ void __launch_bounds__(SOLVE_WG)
           stf(tn, C + (OffT)s * sizeof(double), k < nx ? y : 0.0);
         }
   }
+}
+
+// ---------------------------------------------------------------- fused substep update
+// k_substep_update: the work of k_stencil_rhs and k_column_solve in one pass,
+// so that b' never leaves the chip.  A block owns one whole y column (all
+// stored positions) and marches through zplanes consecutive z planes: nws =
+// ceil(ldx/64) stencil waves hold one stored position per thread, and one
+// solver wave follows one plane behind.
+//   stencil waves, plane i: publish plane i's vx, vy, vz, T, pr in LDS,
+//     where the k-1 / k+1 values are read; compute the explicit terms,
+//     right-hand sides and history stores exactly as k_stencil_rhs, and write
+//     b' of plane i into LDS.  The loads run one plane ahead: during plane i
+//     the thread fetches plane i+2's own-column values and plane i+1's y
+//     neighbours, histories and stale row, so a whole plane of work covers
+//     their latency (z neighbours stay in registers, as in k_stencil_rhs).
+//   solver wave, during plane i: takes plane i-1's b' from LDS into
+//     registers, 16 lanes per system in the chunk layout of k_column_solve,
+//     solves vx, vy, vz together and then the temperature, whose row 1 takes
+//     the vz increment at k = 1, and writes u* and the new temperature to the
+//     b buffers.
+// The plane values and b' are double-buffered by plane parity, so one barrier
+// per plane orders everything.  The new temperature goes to the fourth b
+// buffer rather than to the field, which blocks of neighbouring columns still
+// read; the correction pass copies it over.  The vz increment at k = 1
+// becomes the column's stale row.  The coefficient rows of the stencil (eight
+// metric rows; scale, dl, du of the three systems) and the L, RINV, E rows of
+// the solves are staged in LDS.  Each affine recurrence of a solve runs per
+// chunk from a zero carry, the chunk maps are scanned over the 16 lanes, and a
+// second pass applies the carry (its running products are k_column_solve's G
+// and H rows).
+constexpr int FU_MAX_LDX = 448;   // at most 7 stencil waves + the solver wave
+enum FuCoef {
+  FC_UDX3C = 0, FC_UDX3M, FC_AMC, FC_ACC, FC_APC, FC_AMS, FC_ACS, FC_APS,
+  FC_XSC, FC_XDL, FC_XDU, FC_YSC, FC_YDL, FC_YDU, FC_TSC, FC_TDL, FC_TDU, FC_N
+};
+__host__ __device__ inline int fu_ldb(const Geom &g) { return g.ldx + 4; }   // padded row of the b' and factor rows
+static size_t substep_update_lds(const Geom &g) {
+  const int nts = 64 * ((g.ldx + 63) / 64);
+  return sizeof(double) * ((size_t)(FC_N + 9 + 2 * 4) * fu_ldb(g) + 2 * 5 * (size_t)nts);
+}
+
+// One system's chunk in registers: lane p of 16 holds rows k0 .. k0+L-1, and
+// its factor rows L, RINV, E in registers too (rl, rr, re).  Rows past the
+// system (k >= n) must come with L = 0, RINV = 0, E = 0 and b = 0, which
+// makes them x = 0 and cuts them off from the real rows (the last real row has
+// no superdiagonal), so every lane runs the same L steps.
+template <int LM>
+__device__ __forceinline__ void reg_chunk_solve(double (&b)[LM], const double (&rl)[LM], const double (&rr)[LM],
+                                                const double (&re)[LM], int L, int p) {
+  double acc = 0.0, A = 1.0;
+#pragma unroll
+  for (int t = 0; t < LM; t++) {
+    if (t < L) {
+      acc = b[t] - rl[t] * acc;
+      b[t] = acc;
+      A *= -rl[t];
+    }
+  }
+  const double cin = scan_up(A, acc, p);
+  double gg = 1.0;
+#pragma unroll
+  for (int t = 0; t < LM; t++) {
+    if (t < L) {
+      gg *= -rl[t];
+      b[t] = b[t] + gg * cin;
+    }
+  }
+  acc = 0.0;
+  A = 1.0;
+#pragma unroll
+  for (int t = LM - 1; t >= 0; t--) {
+    if (t < L) {
+      acc = b[t] * rr[t] - re[t] * acc;
+      b[t] = acc;
+      A *= -re[t];
+    }
+  }
+  const double cin2 = scan_down(A, acc, p);
+  double hh = 1.0;
+#pragma unroll
+  for (int t = LM - 1; t >= 0; t--) {
+    if (t < L) {
+      hh *= -re[t];
+      b[t] = b[t] + hh * cin2;
+    }
+  }
+}
+
+// The values a stencil thread loads for one plane ahead of its use.
+struct FuAhead {
+  double vx, vy, vz, t, pr, vzm;                         // own column, plane i+1 (k, and vz at k-1)
+};
+struct FuSide {
+  double vxjp, vyjp, vyjpm, vzjp, tjp;                   // column j+1, plane i
+  double vxjm, vxjmp, vyjm, vzjm, tjm, prjm;             // column j-1, plane i
+  double vzjmip;                                         // column j-1, plane i+1
+  double rx, ry, rz, rt, st;                             // histories and stale row, plane i
+};
+
+template <int LM, bool USE_RU, bool DEV, bool H32>
+__global__ void __launch_bounds__(512)
+    k_substep_update(FusedArgs a, const double *__restrict__ vx, const double *__restrict__ vy,
+                     const double *__restrict__ vz, const double *__restrict__ T, const double *__restrict__ pr,
+                     double *__restrict__ rux, double *__restrict__ ruy, double *__restrict__ ruz,
+                     double *__restrict__ rut, double *__restrict__ ux, double *__restrict__ uy,
+                     double *__restrict__ uz, double *__restrict__ ut, double *__restrict__ stalep,
+                     const double *__restrict__ kc, const double *__restrict__ facp) {
+  using OffT = uint32_t;
+  extern __shared__ double sh[];
+  const Geom g = a.g;
+  const int L = g.L, ldx = g.ldx, nx = g.nx, nxm = g.nxm;
+  const int nws = (ldx + 63) / 64;
+  const int nts = nws * 64;
+  const int ldb = fu_ldb(g);
+  double *cf = sh;                    // [FC_N][ldb] stencil coefficient rows
+  double *rw = cf + FC_N * ldb;       // [SYS_N][3][ldb] L, RINV, E rows of the solves
+  double *bq = rw + 9 * ldb;          // [2 parities][4][ldb] b' of vx, vy, vz, T
+  double *pub = bq + 8 * ldb;         // [2 parities][5][nts] vx, vy, vz, T, pr of the plane
+  const int zplanes = a.zm > 0 ? a.zm : ZM_DEFAULT;
+  const int nib = (g.nzm + zplanes - 1) / zplanes;
+  const int64_t tile = xcd_block((int64_t)nib * g.nym);
+  if (tile < 0) return;
+  if (DEV && coh_int(&a.ds->halted)) return;
+  const int jc = (int)(tile % g.nym), ib = (int)(tile / g.nym);
+  const int i0 = ib * zplanes, i1 = min(i0 + zplanes, g.nzm);
+  for (int idx = threadIdx.x; idx < (FC_N + 9) * ldx; idx += blockDim.x) {
+    const int r = idx / ldx, sl = idx - r * ldx;
+    double v;
+    if (r < FC_XSC) {
+      const int rows[8] = {KC_UDX3C, KC_UDX3M, KC_AM3CK, KC_AC3CK, KC_AP3CK, KC_AM3SK, KC_AC3SK, KC_AP3SK};
+      v = kc[rows[r] * ldx + sl];
+    } else if (r < FC_N) {
+      const int q = r - FC_XSC, sys = q / 3, f = q - sys * 3;
+      v = facp[(sys * FR_N + (f == 0 ? FR_SCALE : (f == 1 ? FR_DL : FR_DU))) * ldx + sl];
+    } else {   // rows past the system are cut off (see reg_chunk_solve)
+      const int q = r - FC_N, sys = q / 3, f = q - sys * 3;
+      const int nsys = sys == SYS_YZ ? nxm : nx;
+      v = k_of_pos(sl, L) < nsys ? facp[(sys * FR_N + (f == 0 ? FR_L : (f == 1 ? FR_RINV : FR_E))) * ldx + sl]
+                                 : 0.0;
+    }
+    cf[r * ldb + sl] = v;
+  }
+  __syncthreads();
+  const OffT cb = (OffT)ldx * sizeof(double);
+  auto colb = [&](int jj, int ii) { return ((OffT)ii * (OffT)g.nym + (OffT)jj) * cb; };
+  const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  if (w == nws) {
+    // ---- solver wave, one plane behind the stencil waves
+    const int lane = (int)threadIdx.x & 63;
+    const int grp = lane >> 4, p = lane & (P - 1);
+    const int sys = grp == 0 ? SYS_X : (grp == 3 ? SYS_T : SYS_YZ);
+    const int nsys = (grp == 1 || grp == 2) ? nxm : nx;
+    const int k0 = p * L;
+    const int cnt = max(0, min(L, nsys - k0));   // rows of this chunk inside the system
+    // this lane's factor rows, held in registers for the whole z march
+    double rl[LM], rr[LM], re[LM];
+    {
+      const double *rL = rw + (sys * 3 + 0) * ldb, *rR = rw + (sys * 3 + 1) * ldb, *rE = rw + (sys * 3 + 2) * ldb;
+#pragma unroll
+      for (int t = 0; t < LM; t++) {
+        const int sl = min(t, L - 1) * P + p;
+        rl[t] = rL[sl];
+        rr[t] = rR[sl];
+        re[t] = rE[sl];
+      }
+    }
+    double *outp = grp == 0 ? ux : (grp == 1 ? uy : (grp == 2 ? uz : ut));
+    auto solve_store = [&](int icp) {
+      const double *bsrc = bq + ((icp & 1) * 4 + grp) * ldb;
+      double b[LM];
+#pragma unroll
+      for (int t = 0; t < LM; t++) {
+        const double v = bsrc[min(t, L - 1) * P + p];
+        b[t] = t < cnt ? v : 0.0;
+      }
+      const OffT C = colb(jc, icp);
+      double dvz0 = 0.0;
+      if (grp < 3) {   // vx, vy, vz
+        reg_chunk_solve<LM>(b, rl, rr, re, L, p);
+#pragma unroll
+        for (int t = 0; t < LM; t++)
+          if (t < L) {
+            const int k = k0 + t, sl = t * P + p;
+            if (grp == 0) {
+              stf(outp, C + (OffT)sl * sizeof(double), (k >= 1 && k <= nx - 2) ? b[t] : 0.0);
+            } else if (k != nx - 1) {   // row nx is outside the nxm system: never written
+              stf(outp, C + (OffT)sl * sizeof(double), k < nxm ? b[t] : 0.0);
+            }
+          }
+        if (grp == 2 && p == 0) dvz0 = b[0] - coh_dbl(vz + (size_t)C / sizeof(double));   // k = 1: position 0
+      }
+      dvz0 = __shfl(dvz0, 32, 64);   // lane 32: the vz system's chunk 0
+      if (grp == 3) {   // temperature; row 1 with rhs(1) = the vz increment
+        if (p == 0) b[0] += dvz0;
+        reg_chunk_solve<LM>(b, rl, rr, re, L, p);
+#pragma unroll
+        for (int t = 0; t < LM; t++)
+          if (t < L) {
+            const int k = k0 + t, sl = t * P + p;
+            double y = b[t];
+            y = k == 0 ? a.sc.tempbp : y;
+            y = k == nx - 1 ? a.sc.temptp : y;
+            stf(outp, C + (OffT)sl * sizeof(double), k < nx ? y : 0.0);
+          }
+      }
+      if (lane == 0) stalep[(int64_t)icp * g.nym + jc] = dvz0;
+    };
+    for (int ic = i0; ic < i1; ic++) {
+      __syncthreads();   // plane ic published; b' of plane ic-1 complete
+      if (ic > i0) solve_store(ic - 1);
+    }
+    __syncthreads();     // b' of the last plane complete
+    solve_store(i1 - 1);
+    return;
+  }
+  // ---- stencil waves: thread = stored position
+  const StepScalars c = DEV ? coh_scalars(&a.ds->sc[a.ns - 1]) : a.sc;
+  const bool keep_ru = a.keep_ru != 0;
+  const int tid = (int)threadIdx.x;
+  const int s = min(tid, ldx - 1);
+  const int k = k_of_pos(s, L);
+  const bool live = tid < ldx && k < nx;
+  const int sm = k >= 1 ? pos_of_k(k - 1, L) : s;
+  const int sp = k + 1 < nx ? pos_of_k(k + 1, L) : s;
+  const OffT sb = (OffT)s * sizeof(double), smb = (OffT)sm * sizeof(double), spb = (OffT)sp * sizeof(double);
+  const bool inx = k >= 1 && k <= nx - 2;
+  const bool iny = k <= nxm - 1;
+  const int jm = wrapm(jc - 1, g.nym), jp = wrapp(jc + 1, g.nym);
+  auto cfv = [&](int r) { return cf[r * ldb + s]; };
+  auto load_ahead = [&](int ii) {
+    const OffT CI = colb(jc, ii);
+    FuAhead n;
+    n.vx = ldf(vx, CI + sb); n.vy = ldf(vy, CI + sb); n.vz = ldf(vz, CI + sb); n.t = ldf(T, CI + sb);
+    n.pr = ldf(pr, CI + sb); n.vzm = ldf(vz, CI + smb);
+    return n;
+  };
+  auto load_side = [&](int ii) {   // plane ii's y neighbours, histories and stale row
+    const int iip = wrapp(ii + 1, g.nzm);
+    const OffT CJP = colb(jp, ii), CJM = colb(jm, ii), C = colb(jc, ii);
+    FuSide y;
+    y.vxjp = ldf(vx, CJP + sb); y.vyjp = ldf(vy, CJP + sb); y.vyjpm = ldf(vy, CJP + smb);
+    y.vzjp = ldf(vz, CJP + sb); y.tjp = ldf(T, CJP + sb);
+    y.vxjm = ldf(vx, CJM + sb); y.vxjmp = ldf(vx, CJM + spb); y.vyjm = ldf(vy, CJM + sb);
+    y.vzjm = ldf(vz, CJM + sb); y.tjm = ldf(T, CJM + sb); y.prjm = ldf(pr, CJM + sb);
+    y.vzjmip = ldf(vz, colb(jm, iip) + sb);
+    y.rx = USE_RU ? ldh<H32>(rux, C + sb) : 0.0;
+    y.ry = USE_RU ? ldh<H32>(ruy, C + sb) : 0.0;
+    y.rz = USE_RU ? ldh<H32>(ruz, C + sb) : 0.0;
+    y.rt = USE_RU ? ldh<H32>(rut, C + sb) : 0.0;
+    y.st = (k == 0 && tid < ldx) ? coh_dbl(stalep + (int64_t)ii * g.nym + jc) : 0.0;
+    return y;
+  };
+  // plane i0 (centre), i0-1 (z neighbours carried in registers) and i0+1
+  const OffT C0 = colb(jc, i0);
+  double vxc = ldf(vx, C0 + sb), vyc = ldf(vy, C0 + sb), vzc = ldf(vz, C0 + sb), tc = ldf(T, C0 + sb),
+         prc = ldf(pr, C0 + sb);
+  const int ib0 = wrapm(i0 - 1, g.nzm);
+  const OffT CM = colb(jc, ib0);
+  double vxim = ldf(vx, CM + sb), vximp = ldf(vx, CM + spb), vyim = ldf(vy, CM + sb);
+  double vzim = ldf(vz, CM + sb), tim = ldf(T, CM + sb), prim = ldf(pr, CM + sb);
+  double vyjpim = ldf(vy, colb(jp, ib0) + sb);
+  FuAhead nx1 = load_ahead(wrapp(i0 + 1, g.nzm));
+  FuSide y0 = load_side(i0);
+  for (int ic = i0; ic < i1; ic++) {
+    const int ip2 = wrapp(wrapp(ic + 1, g.nzm) + 1, g.nzm);
+    const FuAhead nx2 = load_ahead(ip2);                 // plane ic+2
+    const FuSide y1 = load_side(wrapp(ic + 1, g.nzm));  // plane ic+1
+    const OffT C = colb(jc, ic);
+    double *own = pub + (ic & 1) * 5 * nts;
+    own[0 * nts + tid] = vxc;
+    own[1 * nts + tid] = vyc;
+    own[2 * nts + tid] = vzc;
+    own[3 * nts + tid] = tc;
+    own[4 * nts + tid] = prc;
+    __syncthreads();   // plane ic published; the solver takes plane ic-1's b'
+    const double vxm = own[0 * nts + sm], vxp = own[0 * nts + sp];
+    const double vym = own[1 * nts + sm], vyp = own[1 * nts + sp];
+    const double vzm = own[2 * nts + sm], vzp = own[2 * nts + sp];
+    const double tm = own[3 * nts + sm], tp = own[3 * nts + sp];
+    const double prm = own[4 * nts + sm];
+    const double vxip = nx1.vx, vyip = nx1.vy, vzip = nx1.vz, vzipm = nx1.vzm, tip = nx1.t;
+    const double vxjp = y0.vxjp, vyjp = y0.vyjp, vyjpm = y0.vyjpm, vzjp = y0.vzjp, tjp = y0.tjp;
+    const double vxjm = y0.vxjm, vxjmp = y0.vxjmp, vyjm = y0.vyjm, vzjm = y0.vzjm, tjm = y0.tjm;
+    const double prjm = y0.prjm, vzjmip = y0.vzjmip;
+    double *bqc = bq + (ic & 1) * 4 * ldb;
+    if (live) {
+      const double udx3c = cfv(FC_UDX3C), udx3m = cfv(FC_UDX3M);
+      // ---- vx (ExplicitTermsVX.F90:55-101, ImplicitAndUpdateVX.F90:47-60)
+      {
+        double hxz = (((vzip + vzipm) * (vxip + vxc)) - ((vzc + vzm) * (vxc + vxim))) * c.udz4;
+        double hxy = (((vyjp + vyjpm) * (vxjp + vxc)) - ((vyc + vym) * (vxc + vxjm))) * c.udy4;
+        double aa = vxp + vxc, bb = vxc + vxm;
+        double hxx = (aa * aa - bb * bb) * udx3c * 0.25;
+        double dzzvx = (vxim - 2.0 * vxc + vxip) * c.udzq_re;
+        double dyyvx = (vxjm - 2.0 * vxc + vxjp) * c.udyq_re;
+        double q = -(hxx + hxy + hxz) + dyyvx + dzzvx + tc;
+        double dxxvx = vxp * cfv(FC_APC) + vxc * cfv(FC_ACC) + vxm * cfv(FC_AMC);
+        double dxp = (prc - prm) * (c.al * udx3c);
+        double r = USE_RU ? c.ro * y0.rx : 0.0;
+        double rhs = (c.ga * q + r + c.alre * dxxvx - dxp) * c.dt;
+        double vin = rhs * cfv(FC_XSC) + ((cfv(FC_XDL) * vxm + vxc) + cfv(FC_XDU) * vxp);
+        if (keep_ru) sth<H32>(rux, C + sb, inx ? q : 0.0);
+        bqc[0 * ldb + s] = inx ? vin : y0.st + vxc;
+      }
+      // ---- vy (ExplicitTermsVY.F90:52-98, ImplicitAndUpdateVY.F90:52-68); kmv/kpv clamps
+      {
+        const double vymm = k == 0 ? vyc : vym, vypp = k == nxm - 1 ? vyc : vyp;
+        double hyx = ((vxp + vxjmp) * (vypp + vyc) - (vxc + vxjm) * (vyc + vymm)) * udx3m * 0.25;
+        double yp = vyjp + vyc, ym = vyjm + vyc;
+        double hyy = (yp * yp - ym * ym) * c.udy4;
+        double hyz = ((vyip + vyc) * (vzip + vzjmip) - (vyc + vyim) * (vzc + vzjm)) * c.udz4;
+        double dyyvy = (vyjp - 2.0 * vyc + vyjm) * c.udyq_re;
+        double dzzvy = (vyip - 2.0 * vyc + vyim) * c.udzq_re;
+        double h = -(hyx + hyy + hyz) + dyyvy + dzzvy;
+        double dxxvy = vypp * cfv(FC_APS) + vyc * cfv(FC_ACS) + vymm * cfv(FC_AMS);
+        double dyp = (prc - prjm) * c.udy_al;
+        double r = USE_RU ? c.ro * y0.ry : 0.0;
+        double rhs = (c.ga * h + r + c.alre * dxxvy - dyp) * c.dt;
+        if (keep_ru) sth<H32>(ruy, C + sb, iny ? h : 0.0);
+        bqc[1 * ldb + s] = rhs * cfv(FC_YSC) + ((cfv(FC_YDL) * vym + vyc) + cfv(FC_YDU) * vyp);
+      }
+      // ---- vz (ExplicitTermsVZ.F90:56-101, ImplicitAndUpdateVZ.F90:48-64)
+      {
+        const double vzmm = k == 0 ? vzc : vzm, vzpp = k == nxm - 1 ? vzc : vzp;
+        double zp = vzip + vzc, zm = vzim + vzc;
+        double hzz = (zp * zp - zm * zm) * c.udz4;
+        double hzy = ((vyjp + vyjpim) * (vzjp + vzc) - (vyc + vyim) * (vzc + vzjm)) * c.udy4;
+        double hzx = ((vxp + vximp) * (vzpp + vzc) - (vxc + vxim) * (vzc + vzmm)) * udx3m * 0.25;
+        double dzzvz = (vzip - 2.0 * vzc + vzim) * c.udzq_re;
+        double dyyvz = (vzjp - 2.0 * vzc + vzjm) * c.udyq_re;
+        double h = -(hzx + hzy + hzz) + dyyvz + dzzvz;
+        double dxxvz = vzpp * cfv(FC_APS) + vzc * cfv(FC_ACS) + vzmm * cfv(FC_AMS);
+        double dzp = (prc - prim) * c.dz * c.al;
+        double r = USE_RU ? c.ro * y0.rz : 0.0;
+        double rhs = (c.ga * h + r + c.alre * dxxvz - dzp) * c.dt;
+        if (keep_ru) sth<H32>(ruz, C + sb, iny ? h : 0.0);
+        bqc[2 * ldb + s] = rhs * cfv(FC_YSC) + ((cfv(FC_YDL) * vzm + vzc) + cfv(FC_YDU) * vzp);
+      }
+      // ---- temperature (ExplicitTermsTemp.F90:52-93, ImplicitAndUpdateTemp.F90:41-54)
+      {
+        double htz = ((vzipm + vzip) * (tip + tc) - (vzm + vzc) * (tc + tim)) * c.udz4;
+        double hty = ((vyjp + vyjpm) * (tjp + tc) - (vyc + vym) * (tc + tjm)) * c.udy4;
+        double htx = ((vxp + vxc) * (tp + tc) - (vxc + vxm) * (tc + tm)) * udx3c * 0.25;
+        double dzzt = (tip - 2.0 * tc + tim) * c.udzq_pe;
+        double dyyt = (tjp - 2.0 * tc + tjm) * c.udyq_pe;
+        double h = -(htx + hty + htz) + dyyt + dzzt;
+        double dxxt = tp * cfv(FC_APC) + tc * cfv(FC_ACC) + tm * cfv(FC_AMC);
+        double r = USE_RU ? c.ro * y0.rt : 0.0;
+        double rhs = (c.ga * h + r + c.alpec * dxxt) * c.dt;
+        double vin = rhs * cfv(FC_TSC) + ((cfv(FC_TDL) * tm + tc) + cfv(FC_TDU) * tp);
+        if (keep_ru) sth<H32>(rut, C + sb, inx ? h : 0.0);
+        bqc[3 * ldb + s] = inx ? vin : tc;   // row 1 gets the vz increment added by the solver
+      }
+    }
+    vxim = vxc; vximp = vxp; vyim = vyc; vzim = vzc; tim = tc; prim = prc;
+    vyjpim = vyjp;
+    vxc = nx1.vx; vyc = nx1.vy; vzc = nx1.vz; tc = nx1.t; prc = nx1.pr;
+    nx1 = nx2;
+    y0 = y1;
+  }
+  __syncthreads();     // b' of the last plane complete
 }
 
 // ---------------------------------------------------------------- per-routine kernels
@@ -787,11 +1149,11 @@ __global__ void __launch_bounds__(SY_WG) k_correct(CorrectArgs a, int which, int
   const int ngr = (g.nym + SY_COLS - 1) / SY_COLS;
   const int nib = (g.nzm + zplanes - 1) / zplanes;
   const int64_t tile = xcd_block((int64_t)nib * ngr * nq);
-  if (tile < 0 || (a.ds && a.ds->halted)) return;
+  if (tile < 0 || (a.ds && coh_int(&a.ds->halted))) return;
   const int q = (int)(tile % nq);
   const int64_t rest = tile / nq;
   const int gr = (int)(rest % ngr), ib = (int)(rest / ngr);
-  const StepScalars c = a.ds ? a.ds->sc[a.ns - 1] : a.sc;
+  const StepScalars c = a.ds ? coh_scalars(&a.ds->sc[a.ns - 1]) : a.sc;
   for (int idx = threadIdx.x; idx < CZ_ROWS * 64; idx += SY_WG) {
     const int r = idx / 64, sl = min(q * 64 + idx - r * 64, ldx - 1);
     int sp = sl;
@@ -855,6 +1217,7 @@ __global__ void __launch_bounds__(SY_WG) k_correct(CorrectArgs a, int which, int
       if (!jp_in) uyjp = uy[col_off(jp, ic, g) + s];
     }
     if (which & 2) pro = qp[C + s];
+    const double tnew = a.ut && colok && sraw < ldx ? a.ut[C + s] : 0.0;
     double *buf = xv + par * (2 * SY_WG);
     buf[tid] = d;
     if (CFL) buf[SY_WG + tid] = vyo;
@@ -885,6 +1248,7 @@ __global__ void __launch_bounds__(SY_WG) k_correct(CorrectArgs a, int which, int
                                     (dkp * app + d * acc + dkm * amm));
     }
     if (top && (which & 1)) qx[C + s] = vxo;
+    if (a.ut && colok && sraw < ldx) a.temp[C + s] = tnew;   // every stored position, padding included
     dim_ = d;
     d = dip;
     uzc = uzip;
@@ -934,6 +1298,16 @@ hipError_t column_configure_lds(size_t bytes) {
   SETC(uint32_t, true) SETC(uint32_t, false) SETC(uint64_t, true) SETC(uint64_t, false)
 #undef SETC
 #undef SETC1
+#define SETU1(M, R, D, H)                                                                                \
+  e = hipFuncSetAttribute((const void *)k_substep_update<M, R, D, H>,                                   \
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);                       \
+  if (e != hipSuccess) return e;
+#define SETU(M, R) SETU1(M, R, true, false) SETU1(M, R, false, false) SETU1(M, R, true, true) SETU1(M, R, false, true)
+#define SETUM(M) SETU(M, true) SETU(M, false)
+  SETUM(8) SETUM(16) SETUM(25) SETUM(28)
+#undef SETUM
+#undef SETU
+#undef SETU1
   return hipSuccess;
 }
 
@@ -965,6 +1339,39 @@ hipError_t launch_stencil_rhs(const FusedArgs &a, hipStream_t st) {
   return hipErrorInvalidValue;
 }
 
+static int fu_pick_lm(int L) { return L <= 8 ? 8 : (L <= 16 ? 16 : (L <= 25 ? 25 : (L <= 28 ? 28 : -1))); }
+
+bool substep_update_fits(const Geom &g) {
+  return g.ldx <= FU_MAX_LDX && fu_pick_lm(g.L) > 0 && substep_update_lds(g) <= 160 * 1024 &&
+         (uint64_t)g.ncol * g.ldx * sizeof(double) < (1ull << 32);
+}
+
+hipError_t launch_substep_update(const FusedArgs &a, hipStream_t st) {
+  const Geom &g = a.g;
+  if (!substep_update_fits(g)) return hipErrorInvalidValue;
+  const int zplanes = a.zm > 0 ? a.zm : ZM_DEFAULT;
+  const int64_t ntile = (int64_t)((g.nzm + zplanes - 1) / zplanes) * g.nym;
+  const dim3 grid((unsigned)(((ntile + 7) / 8) * 8));
+  const dim3 block((unsigned)(64 * ((g.ldx + 63) / 64 + 1)));
+  const size_t shm = substep_update_lds(g);
+  const bool ru = a.sc.use_ru != 0, dev = a.ds != nullptr, h32 = a.h32 != 0;
+  const int lm = fu_pick_lm(g.L);
+#define SU1(M, R, D, H)                                                                                  \
+  if (M == lm && R == ru && D == dev && H == h32) {                                                     \
+    hipLaunchKernelGGL((k_substep_update<M, R, D, H>), grid, block, shm, st, a, a.f.vx, a.f.vy, a.f.vz,   \
+                       a.f.temp, a.f.pr, a.rux, a.ruy, a.ruz, a.rut, a.bx, a.by, a.bz, a.bt, a.stale, a.kc,  \
+                       a.fac);                                                                         \
+    return hipGetLastError();                                                                          \
+  }
+#define SU(M, R) SU1(M, R, true, false) SU1(M, R, false, false) SU1(M, R, true, true) SU1(M, R, false, true)
+#define SUM(M) SU(M, true) SU(M, false)
+  SUM(8) SUM(16) SUM(25) SUM(28)
+#undef SUM
+#undef SU
+#undef SU1
+  return hipErrorInvalidValue;
+}
+
 hipError_t launch_column_solve(const FusedArgs &a, hipStream_t st) {
   const bool small = (uint64_t)a.g.ncol * a.g.ldx * sizeof(double) < (1ull << 32);
   const int64_t nblk = (a.g.ncol + SOLVE_COLS - 1) / SOLVE_COLS;
@@ -975,10 +1382,10 @@ hipError_t launch_column_solve(const FusedArgs &a, hipStream_t st) {
   case V:                                                                                              \
     if (small)                                                                                         \
       hipLaunchKernelGGL((k_column_solve<V, uint32_t>), grid, dim3(SOLVE_WG), shm, st, a, a.f.vz, a.bx, \
-                         a.by, a.bz, a.bt, a.tn, a.stale, a.fac);                                        \
+                         a.by, a.bz, a.bt, a.ux, a.uy, a.uz, a.tn, a.stale, a.fac);                      \
     else                                                                                               \
       hipLaunchKernelGGL((k_column_solve<V, uint64_t>), grid, dim3(SOLVE_WG), shm, st, a, a.f.vz, a.bx, \
-                         a.by, a.bz, a.bt, a.tn, a.stale, a.fac);                                        \
+                         a.by, a.bz, a.bt, a.ux, a.uy, a.uz, a.tn, a.stale, a.fac);                      \
     return hipGetLastError();
     AFID_LM_LIST(CASE)
 #undef CASE
@@ -1027,6 +1434,10 @@ hipError_t launch_correct(const CorrectArgs &a, int which, hipStream_t st) {
 // LDS per block: WG*L doubles (per-routine solve), SC_N*64 + 2*SV_N*SY_WG (stencil
 // pass), SYS_N*RW_N*16*L (column solve); L <= 80 keeps all within gfx950's 160 KiB.
 int max_supported_L() { return 80; }
-size_t column_lds_need(const Geom &g) { return std::max(stencil_lds(g), solve_lds(g)); }
+size_t column_lds_need(const Geom &g) {
+  size_t b = std::max(stencil_lds(g), solve_lds(g));
+  if (substep_update_fits(g)) b = std::max(b, substep_update_lds(g));
+  return b;
+}
 
 }  // namespace afid

@@ -203,8 +203,8 @@ __global__ void __launch_bounds__(256) k_yfft_forward(PoissonArgs a) {
   const int ntile = g.ldx / TK;
   const int64_t tile = fft_tile((int64_t)ntile * g.nzm);
   if (tile < 0) return;
-  if (a.ds && a.ds->halted) return;
-  const StepScalars sc = a.ds ? a.ds->sc[a.ns - 1] : a.sc;
+  if (a.ds && coh_int(&a.ds->halted)) return;
+  const StepScalars sc = a.ds ? coh_scalars(&a.ds->sc[a.ns - 1]) : a.sc;
   const int s0 = (int)(tile % ntile) * TK, i = (int)(tile / ntile);
   const bool hsw = (H & 7) == 0;
   const double usdtal = 1.0 / (sc.dt * sc.al);
@@ -297,7 +297,7 @@ __global__ void __launch_bounds__(256) k_zfft(PoissonArgs a, R sign) {
   const int N = a.pz.n;
   const int ntile = (a.nymh + TM - 1) / TM;
   const int64_t tile = fft_tile((int64_t)ntile * a.g.nxm);
-  if (tile < 0 || (a.ds && a.ds->halted)) return;
+  if (tile < 0 || (a.ds && coh_int(&a.ds->halted))) return;
   const int m0 = (int)(tile % ntile) * TM, k = (int)(tile / ntile);
   const bool zsw = (N & 7) == 0;
   C *S = reinterpret_cast<C *>(a.S) + (int64_t)k * a.g.nzm * a.ldm;
@@ -413,7 +413,7 @@ __global__ void __launch_bounds__(256) k_xsolve_pressure(PoissonArgs a) {
   const int nymh = a.nymh, nzm = a.g.nzm, nxm = a.g.nxm;
   const int nmb = (nymh + MPB - 1) / MPB;
   const int64_t tile = xcd_block((int64_t)nmb * nzm);
-  if (tile < 0 || (a.ds && a.ds->halted)) return;
+  if (tile < 0 || (a.ds && coh_int(&a.ds->halted))) return;
   for (int i = threadIdx.x; i < nxm; i += 256) {
     xco[i] = a.acphk[i];
     xco[nxm + i] = a.amphk[i];
@@ -546,7 +546,7 @@ __global__ void __launch_bounds__(256) k_yfft_inverse(PoissonArgs a) {
   const int ntile = g.ldx / TK;
   const int64_t tile = fft_tile((int64_t)ntile * g.nzm);
   if (tile < 0) return;
-  if (a.ds && a.ds->halted) return;
+  if (a.ds && coh_int(&a.ds->halted)) return;
   const int s0 = (int)(tile % ntile) * TK, i = (int)(tile / ntile);
   const bool hsw = (H & 7) == 0;
   const int nymh = a.nymh;

@@ -15,22 +15,25 @@
 namespace afid {
 
 __global__ void k_step_begin(DevStep *ds, const double *cfl_out, StepConsts k) {
-  if (threadIdx.x != 0 || ds->halted) return;
-  const double cfl = cfl_out ? cfl_out[0] : __longlong_as_double((long long)ds->cflbits);
+  // every field is read with coherent loads: earlier kernels of the step
+  // graph (and of the previous replay) wrote them
+  if (threadIdx.x != 0 || coh_int(&ds->halted)) return;
+  const double cfl = cfl_out ? coh_dbl(cfl_out) : __longlong_as_double((long long)coh_u64(&ds->cflbits));
   ds->cflbits = 0;   // the last correction of this step accumulates the next step's value
   const double instcfl = fmax(0.00000001, cfl);   // CalcMaxCFL floor
-  double dt = ds->dtfix;
+  const double dtmin = coh_dbl(&ds->dtmin), dtmax = coh_dbl(&ds->dtmax), limitcfl = coh_dbl(&ds->limitcfl);
+  double dt = coh_dbl(&ds->dtfix);
   int err = 0;
-  if (ds->idtv) {
-    if (ds->ntime > 1) {
-      dt = instcfl < 1.0e-8 ? ds->dtmax : ds->limitcfl / instcfl;
-      if (dt > ds->dtmax) dt = ds->dtmax;
+  if (coh_int(&ds->idtv)) {
+    if (coh_int(&ds->ntime) > 1) {
+      dt = instcfl < 1.0e-8 ? dtmax : limitcfl / instcfl;
+      if (dt > dtmax) dt = dtmax;
     } else {
-      dt = ds->dtmin;
+      dt = dtmin;
     }
-    if (dt < ds->dtmin) err = 166;
+    if (dt < dtmin) err = 166;
   } else {
-    if (instcfl * dt > ds->limitcfl) err = 165;
+    if (instcfl * dt > limitcfl) err = 165;
   }
   ds->dt = dt;
   ds->errorcode = err;
@@ -55,11 +58,11 @@ __global__ void k_step_begin(DevStep *ds, const double *cfl_out, StepConsts k) {
 }
 
 __global__ void k_step_end(DevStep *ds) {
-  if (threadIdx.x != 0 || ds->halted) return;
-  ds->time = ds->time + ds->dt;
-  ds->ntime += 1;
-  ds->steps_done += 1;
-  if (ds->errorcode != 0) ds->halted = 1;
+  if (threadIdx.x != 0 || coh_int(&ds->halted)) return;
+  ds->time = coh_dbl(&ds->time) + coh_dbl(&ds->dt);
+  ds->ntime = coh_int(&ds->ntime) + 1;
+  ds->steps_done = coh_int(&ds->steps_done) + 1;
+  if (coh_int(&ds->errorcode) != 0) ds->halted = 1;
 }
 
 // One block.  Per substep and system: coefficients (all threads), LU in
@@ -67,16 +70,16 @@ __global__ void k_step_end(DevStep *ds) {
 // thread per system and chunk), then the interleaved factor rows.
 enum { W_DL = 0, W_DU, W_SC, W_L, W_U, W_E, W_N };
 __global__ void __launch_bounds__(256) k_factor(FactorArgs a) {
-  if (a.ds->halted) return;
+  if (coh_int(&a.ds->halted)) return;
   const Geom g = a.g;
   const int nx = g.nx, nxm = g.nxm, ldx = g.ldx, L = g.L;
   const double *ac3ck = a.mc, *am3ssk = a.mc + nx, *ap3ssk = a.mc + 2 * nx, *ac3sk = a.mc + 3 * nx;
   const double *am3sk = a.mc + 4 * nx, *ap3sk = a.mc + 5 * nx, *ac3ssk = a.mc + 6 * nx;
-  const double dt = a.ds->dt;
+  const double dt = coh_dbl(&a.ds->dt);
   auto W = [&](int q, int r) { return a.work + ((size_t)q * W_N + r) * nx; };
   auto nsys = [&](int q) { return q == SYS_YZ ? nxm : nx; };
   for (int ns = 0; ns < a.k.nsst; ns++) {
-    const double al = a.ds->sc[ns].al;
+    const double al = coh_dbl(&a.ds->sc[ns].al);
     const double beta = dt / a.k.ren * 0.5;
     for (int idx = threadIdx.x; idx < SYS_N * nx; idx += blockDim.x) {
       const int q = idx / nx, k = idx - q * nx, kc = k + 1;

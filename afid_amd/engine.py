@@ -464,6 +464,13 @@ class Cell:
     def reset_stats(self):
         self._check(self.L.afid_reset_stats(self.h), "reset_stats")
 
+    def get_stale(self):
+        """The stale rhs(1) value of every column (the vz increment at k = 1 of
+        the last substep), as (nzm, nym)."""
+        out = np.zeros(self.nym * self.nzm)
+        self._check(self.L.afid_get_stale(self.h, _dp(out)), "get_stale")
+        return out.reshape(self.nzm, self.nym)
+
     def WriteStats(self, directory: str = ".", readstats: bool = False, disscal: bool = True):
         self._check(load_io_library().afid_write_stats(self.h, directory.encode(), int(readstats), int(disscal)),
                     "WriteStats")

@@ -309,6 +309,43 @@ def test_device_steps_match_host_steps(afid, c):
     dev.close()
 
 
+@pytest.mark.parametrize("c", SMALL_CASES, ids=case_id)
+def test_replayed_step_graphs_take_each_steps_cfl(afid, c):
+    """Many steps from one afid_steps call (the step graph replayed back to
+    back, each step's dt from the CFL number the previous step's last
+    correction folded on the device) against host-driven steps that reduce
+    CalcMaxCFL on the host every step.  A small cflmax makes dt = cflmax/CFL
+    change every step, so a step that read a stale CFL number would show in
+    the time and the fields."""
+    ctl = (1, 2e-3, 1e-6, 5e-3, 0.005)
+    host = afid.Cell(afid.CellParams(nsst=3, alx3=1.0, **c))
+    dev = afid.Cell(afid.CellParams(nsst=3, alx3=1.0, **c))
+    rng = np.random.default_rng(11)
+    nzm, nym, nx = c["nzm"], c["nym"], c["nxm"] + 1
+    vel = {}
+    for nm in ("vx", "vy", "vz"):   # a moving start, so that the CFL number sets dt from the third step
+        a = 0.1 * rng.standard_normal((nzm, nym, nx))
+        a[:, :, -1] = 0.0
+        if nm == "vx":
+            a[:, :, 0] = 0.0
+        vel[nm] = a
+    for x in (host, dev):
+        x.init_linear_noise(7, 1e-3)
+        for nm, a in vel.items():
+            x.upload(nm, a)
+    dts = [host.step(n, *ctl)[1] for n in range(14)]
+    assert len(set(dts[2:])) == len(dts[2:])        # every CFL-limited dt differs
+    e, dt, done = dev.steps(0, 14, *ctl)
+    assert (e, done) == (0, 14)
+    assert dt == pytest.approx(dts[-1], rel=1e-12)
+    assert dev.time == pytest.approx(host.time, rel=1e-12)
+    assert dev.steps_cfl() == dev.CalcMaxCFL()
+    for nm in ("vx", "vy", "vz", "temp"):
+        assert rel_l2(dev.download(nm), host.download(nm)) < 1e-13, nm
+    host.close()
+    dev.close()
+
+
 def test_device_steps_stop_after_error_code(afid):
     """dtmax < dtmin: the third step (ntime = 2, CFL dt clipped to dtmax) raises
     166; main.F90 finishes that step and quits, and so does afid_steps."""
