@@ -7,6 +7,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -44,6 +45,7 @@ struct afid_ctx {
   double gam[4] = {0}, rom[4] = {0}, alm[4] = {0};
   double al = 0, ga = 0, ro = 0, dt = 0, beta = 0, time = 0;
   int singular_mode = 0;       // zgttrf info of the (0,0) pressure mode (>0: reference aborts)
+  int dev = -1;                // HIP device the context was created on
   // device state
   double *f[AFID_NFIELDS] = {nullptr};
   double *bp[4] = {nullptr};   // substep update: right-hand sides b + A q_old of vx, vy, vz, temp;
@@ -636,6 +638,8 @@ int afid_destroy(afid_ctx *c) {
   return AFID_OK;
 }
 
+static int scratch_setup(afid_ctx *c);   // the no-allocation scratch of this device
+
 int afid_create(afid_ctx **out, const afid_params *p, const afid_metrics *m) {
   if (!out || !p) return AFID_ERR_ARG;
   *out = nullptr;
@@ -688,6 +692,8 @@ int afid_create(afid_ctx **out, const afid_params *p, const afid_metrics *m) {
     }                           \
   } while (0)
 #define TRYH(x) TRY(kerr(c, (x)))
+  TRYH(hipGetDevice(&c->dev));
+  TRY(scratch_setup(c));
   TRYH(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
   TRYH(hipEventCreateWithFlags(&c->fac_ev, hipEventDisableTiming));
   TRYH(hipEventRecord(c->fac_ev, c->st));
@@ -1580,6 +1586,41 @@ int afid_calc_stats(afid_ctx *c) {
   if (rc) return rc;
   c->nstatsamples += 1;
   return AFID_OK;
+}
+
+// ---- statistics without allocation ----------------------------------------------
+// The block partials of afid_calc_stats_noalloc live in one static device array
+// per GPU.  Host code never uses the array's name as a device address: its
+// address on each device comes from hipGetSymbolAddress, taken once per device
+// (with that device current) by the first afid_create on it, and only that
+// pointer is passed to kernels.
+static __device__ __align__(256) double scratch[AFID_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+constexpr int SCRATCH_MAX_DEV = 64;
+static double *scratch_dev[SCRATCH_MAX_DEV];
+static std::mutex scratch_mu;
+
+static int scratch_setup(afid_ctx *c) {
+  if (c->dev < 0 || c->dev >= SCRATCH_MAX_DEV) return AFID_ERR_ARG;
+  std::lock_guard<std::mutex> lk(scratch_mu);
+  if (scratch_dev[c->dev]) return AFID_OK;
+  void *ptr = nullptr;
+  HIPCHK(c, hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  scratch_dev[c->dev] = static_cast<double *>(ptr);
+  return AFID_OK;
+}
+
+int afid_calc_stats_noalloc(afid_ctx *c, double *acc) {
+  if (!c || !acc) return AFID_ERR_ARG;
+  if (stats_partial_doubles(c->g) * sizeof(double) > AFID_NOALLOC_SCRATCH_BYTES) return AFID_ERR_ARG;
+  double *ptr = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(scratch_mu);
+    if (c->dev >= 0 && c->dev < SCRATCH_MAX_DEV) ptr = scratch_dev[c->dev];
+  }
+  if (!ptr) return AFID_ERR_STATE;   // afid_create on this device did not set it up
+  ReduceArgs a = reduce_args(c);
+  a.partial = ptr;   // fully written by the partial pass before the finish pass reads it
+  return kerr(c, launch_stats(a, acc, c->st));
 }
 
 int afid_get_stats(afid_ctx *c, double *out, int *nsamples) {

@@ -107,6 +107,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "afid_calc_stats": ([vp], i),
         "afid_get_stats": ([vp, pd, ctypes.POINTER(i)], i),
         "afid_reset_stats": ([vp], i),
+        "afid_calc_stats_noalloc": ([vp, vp], i),
         "afid_get_slabs": ([vp, ctypes.POINTER(i), i, pd], i),
         "afid_interp_field": ([vp, i, pd, i, i, i, i, d], i),
         "afid_set_stale": ([vp, pd], i),
@@ -470,6 +471,12 @@ class Cell:
         out = np.zeros(self.nym * self.nzm)
         self._check(self.L.afid_get_stale(self.h, _dp(out)), "get_stale")
         return out.reshape(self.nzm, self.nym)
+
+    def CalcStatsNoAlloc(self, acc_ptr: int):
+        """afid_calc_stats_noalloc: adds this sample's nine plane-average
+        profiles into the caller's device array at acc_ptr (9*nxm doubles);
+        allocates nothing and returns without synchronising."""
+        self._check(self.L.afid_calc_stats_noalloc(self.h, ctypes.c_void_p(acc_ptr)), "CalcStatsNoAlloc")
 
     def WriteStats(self, directory: str = ".", readstats: bool = False, disscal: bool = True):
         self._check(load_io_library().afid_write_stats(self.h, directory.encode(), int(readstats), int(disscal)),

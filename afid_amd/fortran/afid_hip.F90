@@ -372,6 +372,13 @@
           import :: c_int, c_ptr
           type(c_ptr), value :: ctx
         end function
+!       afid_calc_stats without allocation: acc is a device array of
+!       9*nxm doubles (no counterpart in the reference)
+        integer(c_int) function afid_calc_stats_noalloc(ctx, acc)       &
+     &      bind(C, name='afid_calc_stats_noalloc')
+          import :: c_int, c_ptr
+          type(c_ptr), value :: ctx, acc
+        end function
         integer(c_int) function afid_write_stats(ctx, dir, readstats,    &
      &      disscal) bind(C, name='afid_write_stats')
           import :: c_int, c_ptr, c_char

@@ -232,6 +232,21 @@ int afid_calc_stats(afid_ctx *ctx);
  * the number of afid_calc_stats calls. */
 int afid_get_stats(afid_ctx *ctx, double *out, int *nsamples);
 int afid_reset_stats(afid_ctx *ctx);
+/* No counterpart in the reference.  afid_calc_stats for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): it allocates nothing and adds this
+ * sample's plane averages (the nine quantities of afid_calc_stats, in the
+ * order of afid_get_stats' first 9*nxm values) into acc, a device array of
+ * 9*nxm doubles that the caller owns and clears.  The context's own
+ * statistics and sample count are not touched.  The block partials use a
+ * static device array of AFID_NOALLOC_SCRATCH_BYTES per GPU, shared by every
+ * call on that GPU and holding the last call's contents: calls that overlap
+ * on one GPU (contexts on different streams) race.  AFID_ERR_ARG when the
+ * partials, 256*9*ldx doubles with ldx = 16*ceil((nxm+1)/16), exceed that
+ * size (nxm >= 896).  Stream-ordered on the context's stream; returns
+ * without synchronising. */
+#define AFID_NOALLOC_SCRATCH_BYTES (16u << 20)
+int afid_calc_stats_noalloc(afid_ctx *ctx, double *acc);
 /* SlabDumper (SlabDumpRoutines.F90:20-31): the planes k = kslab[m] (1-based,
  * m < nslab) of (vx(k)+vx(k+1))/2, vy, vz and temp, as
  * out[((m*4 + q)*nzm + i-1)*nym + j-1] (4*nslab*nym*nzm values). */
