@@ -653,56 +653,123 @@ enum FuCoef {
   FC_UDX3C = 0, FC_UDX3M, FC_AMC, FC_ACC, FC_APC, FC_AMS, FC_ACS, FC_APS,
   FC_XSC, FC_XDL, FC_XDU, FC_YSC, FC_YDL, FC_YDU, FC_TSC, FC_TDL, FC_TDU, FC_N
 };
-__host__ __device__ inline int fu_ldb(const Geom &g) { return g.ldx + 4; }   // padded row of the b' and factor rows
+__host__ __device__ inline int fu_pick_lm(int L) {
+  return L <= 8 ? 8 : (L <= 16 ? 16 : (L <= 25 ? 25 : (L <= 28 ? 28 : -1)));
+}
+// Row length of the b' and factor rows in LDS: every register slot of a chunk
+// (16 * LM positions, past ldx when L < LM) plus a pad that puts the rows of
+// one plane on different banks.
+__host__ __device__ inline int fu_ldb(const Geom &g) {
+  const int lm = fu_pick_lm(g.L);
+  return (g.ldx > P * lm ? g.ldx : P * lm) + 4;
+}
 static size_t substep_update_lds(const Geom &g) {
   const int nts = 64 * ((g.ldx + 63) / 64);
   return sizeof(double) * ((size_t)(FC_N + 9 + 2 * 4) * fu_ldb(g) + 2 * 5 * (size_t)nts);
 }
 
-// One system's chunk in registers: lane p of 16 holds rows k0 .. k0+L-1, and
-// its factor rows L, RINV, E in registers too (rl, rr, re).  Rows past the
-// system (k >= n) must come with L = 0, RINV = 0, E = 0 and b = 0, which
-// makes them x = 0 and cuts them off from the real rows (the last real row has
-// no superdiagonal), so every lane runs the same L steps.
+// Scans over the 16 lanes of a row through DPP row shifts (no LDS
+// permutes): the combined map of the chunks before (up) or after (down)
+// this lane, applied to a zero carry, as scan_up / scan_down compute it.
+__device__ __forceinline__ double dpp_shr(double v, int n, double fill) {
+  const long long x = __double_as_longlong(v), f = __double_as_longlong(fill);
+  int lo = (int)x, hi = (int)(x >> 32);
+  const int flo = (int)f, fhi = (int)(f >> 32);
+  switch (n) {
+    case 1: lo = __builtin_amdgcn_update_dpp(flo, lo, 0x111, 0xf, 0xf, false);
+            hi = __builtin_amdgcn_update_dpp(fhi, hi, 0x111, 0xf, 0xf, false); break;
+    case 2: lo = __builtin_amdgcn_update_dpp(flo, lo, 0x112, 0xf, 0xf, false);
+            hi = __builtin_amdgcn_update_dpp(fhi, hi, 0x112, 0xf, 0xf, false); break;
+    case 4: lo = __builtin_amdgcn_update_dpp(flo, lo, 0x114, 0xf, 0xf, false);
+            hi = __builtin_amdgcn_update_dpp(fhi, hi, 0x114, 0xf, 0xf, false); break;
+    default: lo = __builtin_amdgcn_update_dpp(flo, lo, 0x118, 0xf, 0xf, false);
+             hi = __builtin_amdgcn_update_dpp(fhi, hi, 0x118, 0xf, 0xf, false); break;
+  }
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double dpp_shl(double v, int n, double fill) {
+  const long long x = __double_as_longlong(v), f = __double_as_longlong(fill);
+  int lo = (int)x, hi = (int)(x >> 32);
+  const int flo = (int)f, fhi = (int)(f >> 32);
+  switch (n) {
+    case 1: lo = __builtin_amdgcn_update_dpp(flo, lo, 0x101, 0xf, 0xf, false);
+            hi = __builtin_amdgcn_update_dpp(fhi, hi, 0x101, 0xf, 0xf, false); break;
+    case 2: lo = __builtin_amdgcn_update_dpp(flo, lo, 0x102, 0xf, 0xf, false);
+            hi = __builtin_amdgcn_update_dpp(fhi, hi, 0x102, 0xf, 0xf, false); break;
+    case 4: lo = __builtin_amdgcn_update_dpp(flo, lo, 0x104, 0xf, 0xf, false);
+            hi = __builtin_amdgcn_update_dpp(fhi, hi, 0x104, 0xf, 0xf, false); break;
+    default: lo = __builtin_amdgcn_update_dpp(flo, lo, 0x108, 0xf, 0xf, false);
+             hi = __builtin_amdgcn_update_dpp(fhi, hi, 0x108, 0xf, 0xf, false); break;
+  }
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double row_scan_up(double A, double B) {
+#pragma unroll
+  for (int off = 1; off < P; off <<= 1) {   // lanes below off in the row take the identity (1, 0)
+    const double A2 = dpp_shr(A, off, 1.0), B2 = dpp_shr(B, off, 0.0);
+    B = A * B2 + B;
+    A = A * A2;
+  }
+  return dpp_shr(B, 1, 0.0);
+}
+__device__ __forceinline__ double row_scan_down(double A, double B) {
+#pragma unroll
+  for (int off = 1; off < P; off <<= 1) {
+    const double A2 = dpp_shl(A, off, 1.0), B2 = dpp_shl(B, off, 0.0);
+    B = A * B2 + B;
+    A = A * A2;
+  }
+  return dpp_shl(B, 1, 0.0);
+}
+
+// One system's chunk in registers: lane p of 16 holds rows k0 .. k0+LM-1.
+// The factor rows come from LDS (rL, rR, rE), each pass's rows loaded into
+// registers in one batch before its recurrence, so the chain of dependent
+// multiply-adds never waits on LDS.  Rows past the system (k >= n) hold
+// L = 0, RINV = 0, E = 0 and b = 0: they become x = 0 and are cut off from the
+// real rows (the last real row has no superdiagonal).  Register slots past
+// the chunk (t >= L) hold L = -1, RINV = 0, E = -1, b = 0, which pass both
+// recurrences' values and chunk products through unchanged.  So every lane
+// runs all LM slots with no branch and no index clamp.
 template <int LM>
-__device__ __forceinline__ void reg_chunk_solve(double (&b)[LM], const double (&rl)[LM], const double (&rr)[LM],
-                                                const double (&re)[LM], int L, int p) {
+__device__ __forceinline__ void reg_chunk_solve(double (&b)[LM], const double *rL, const double *rR,
+                                                const double *rE, int p) {
+  double r1[LM], r2[LM];
+#pragma unroll
+  for (int t = 0; t < LM; t++) r1[t] = rL[t * P + p];
   double acc = 0.0, A = 1.0;
 #pragma unroll
   for (int t = 0; t < LM; t++) {
-    if (t < L) {
-      acc = b[t] - rl[t] * acc;
-      b[t] = acc;
-      A *= -rl[t];
-    }
+    acc = b[t] - r1[t] * acc;
+    b[t] = acc;
+    A *= -r1[t];
   }
-  const double cin = scan_up(A, acc, p);
+  const double cin = row_scan_up(A, acc);
   double gg = 1.0;
 #pragma unroll
   for (int t = 0; t < LM; t++) {
-    if (t < L) {
-      gg *= -rl[t];
-      b[t] = b[t] + gg * cin;
-    }
+    gg *= -r1[t];
+    b[t] = b[t] + gg * cin;
+  }
+#pragma unroll
+  for (int t = 0; t < LM; t++) {
+    r1[t] = rR[t * P + p];
+    r2[t] = rE[t * P + p];
   }
   acc = 0.0;
   A = 1.0;
 #pragma unroll
   for (int t = LM - 1; t >= 0; t--) {
-    if (t < L) {
-      acc = b[t] * rr[t] - re[t] * acc;
-      b[t] = acc;
-      A *= -re[t];
-    }
+    acc = b[t] * r1[t] - r2[t] * acc;
+    b[t] = acc;
+    A *= -r2[t];
   }
-  const double cin2 = scan_down(A, acc, p);
+  const double cin2 = row_scan_down(A, acc);
   double hh = 1.0;
 #pragma unroll
   for (int t = LM - 1; t >= 0; t--) {
-    if (t < L) {
-      hh *= -re[t];
-      b[t] = b[t] + hh * cin2;
-    }
+    hh *= -r2[t];
+    b[t] = b[t] + hh * cin2;
   }
 }
 
@@ -743,10 +810,13 @@ __global__ void __launch_bounds__(512)
   if (DEV && coh_int(&a.ds->halted)) return;
   const int jc = (int)(tile % g.nym), ib = (int)(tile / g.nym);
   const int i0 = ib * zplanes, i1 = min(i0 + zplanes, g.nzm);
-  for (int idx = threadIdx.x; idx < (FC_N + 9) * ldx; idx += blockDim.x) {
-    const int r = idx / ldx, sl = idx - r * ldx;
-    double v;
-    if (r < FC_XSC) {
+  for (int idx = threadIdx.x; idx < (FC_N + 9) * ldb; idx += blockDim.x) {
+    const int r = idx / ldb, sl = idx - r * ldb;
+    double v = 0.0;
+    if (sl >= ldx) {   // register slots past the chunk (solve rows only): L = -1, RINV = 0, E = -1
+      const int f = (r - FC_N) % 3;
+      v = r < FC_N ? 0.0 : (f == 1 ? 0.0 : -1.0);
+    } else if (r < FC_XSC) {
       const int rows[8] = {KC_UDX3C, KC_UDX3M, KC_AM3CK, KC_AC3CK, KC_AP3CK, KC_AM3SK, KC_AC3SK, KC_AP3SK};
       v = kc[rows[r] * ldx + sl];
     } else if (r < FC_N) {
@@ -760,6 +830,7 @@ __global__ void __launch_bounds__(512)
     }
     cf[r * ldb + sl] = v;
   }
+  for (int idx = threadIdx.x; idx < 8 * ldb; idx += blockDim.x) bq[idx] = 0.0;   // padding stays 0
   __syncthreads();
   const OffT cb = (OffT)ldx * sizeof(double);
   auto colb = [&](int jj, int ii) { return ((OffT)ii * (OffT)g.nym + (OffT)jj) * cb; };
@@ -769,68 +840,47 @@ __global__ void __launch_bounds__(512)
     const int lane = (int)threadIdx.x & 63;
     const int grp = lane >> 4, p = lane & (P - 1);
     const int sys = grp == 0 ? SYS_X : (grp == 3 ? SYS_T : SYS_YZ);
-    const int nsys = (grp == 1 || grp == 2) ? nxm : nx;
     const int k0 = p * L;
-    const int cnt = max(0, min(L, nsys - k0));   // rows of this chunk inside the system
-    // this lane's factor rows, held in registers for the whole z march
-    double rl[LM], rr[LM], re[LM];
-    {
-      const double *rL = rw + (sys * 3 + 0) * ldb, *rR = rw + (sys * 3 + 1) * ldb, *rE = rw + (sys * 3 + 2) * ldb;
-#pragma unroll
-      for (int t = 0; t < LM; t++) {
-        const int sl = min(t, L - 1) * P + p;
-        rl[t] = rL[sl];
-        rr[t] = rR[sl];
-        re[t] = rE[sl];
-      }
-    }
+    const double *rL = rw + (sys * 3 + 0) * ldb, *rR = rw + (sys * 3 + 1) * ldb, *rE = rw + (sys * 3 + 2) * ldb;
     double *outp = grp == 0 ? ux : (grp == 1 ? uy : (grp == 2 ? uz : ut));
-    auto solve_store = [&](int icp) {
+    for (int ic = i0; ic <= i1; ic++) {
+      __syncthreads();   // plane ic published (ic < i1); b' of plane ic-1 complete
+      if (ic == i0) continue;
+      const int icp = ic - 1;
       const double *bsrc = bq + ((icp & 1) * 4 + grp) * ldb;
       double b[LM];
 #pragma unroll
-      for (int t = 0; t < LM; t++) {
-        const double v = bsrc[min(t, L - 1) * P + p];
-        b[t] = t < cnt ? v : 0.0;
-      }
+      for (int t = 0; t < LM; t++) b[t] = bsrc[t * P + p];   // 0 past the system and the chunk
       const OffT C = colb(jc, icp);
       double dvz0 = 0.0;
-      if (grp < 3) {   // vx, vy, vz
-        reg_chunk_solve<LM>(b, rl, rr, re, L, p);
+      // phase 0: vx, vy, vz; phase 1: temperature, whose row 1 takes the vz increment
+      for (int ph = 0; ph < 2; ph++) {
+        if (ph == 1) {
+          dvz0 = __shfl(dvz0, 32, 64);   // lane 32: the vz system's chunk 0
+          if (grp == 3 && p == 0) b[0] += dvz0;
+        }
+        if (ph == 0 ? grp == 3 : grp != 3) continue;
+        reg_chunk_solve<LM>(b, rL, rR, rE, p);
 #pragma unroll
         for (int t = 0; t < LM; t++)
           if (t < L) {
             const int k = k0 + t, sl = t * P + p;
+            double v;
             if (grp == 0) {
-              stf(outp, C + (OffT)sl * sizeof(double), (k >= 1 && k <= nx - 2) ? b[t] : 0.0);
-            } else if (k != nx - 1) {   // row nx is outside the nxm system: never written
-              stf(outp, C + (OffT)sl * sizeof(double), k < nxm ? b[t] : 0.0);
+              v = (k >= 1 && k <= nx - 2) ? b[t] : 0.0;
+            } else if (grp == 3) {
+              v = k == 0 ? a.sc.tempbp : (k == nx - 1 ? a.sc.temptp : b[t]);
+              v = k < nx ? v : 0.0;
+            } else {
+              if (k == nx - 1) continue;   // row nx is outside the nxm system: never written
+              v = k < nxm ? b[t] : 0.0;
             }
+            stf(outp, C + (OffT)sl * sizeof(double), v);
           }
         if (grp == 2 && p == 0) dvz0 = b[0] - coh_dbl(vz + (size_t)C / sizeof(double));   // k = 1: position 0
       }
-      dvz0 = __shfl(dvz0, 32, 64);   // lane 32: the vz system's chunk 0
-      if (grp == 3) {   // temperature; row 1 with rhs(1) = the vz increment
-        if (p == 0) b[0] += dvz0;
-        reg_chunk_solve<LM>(b, rl, rr, re, L, p);
-#pragma unroll
-        for (int t = 0; t < LM; t++)
-          if (t < L) {
-            const int k = k0 + t, sl = t * P + p;
-            double y = b[t];
-            y = k == 0 ? a.sc.tempbp : y;
-            y = k == nx - 1 ? a.sc.temptp : y;
-            stf(outp, C + (OffT)sl * sizeof(double), k < nx ? y : 0.0);
-          }
-      }
       if (lane == 0) stalep[(int64_t)icp * g.nym + jc] = dvz0;
-    };
-    for (int ic = i0; ic < i1; ic++) {
-      __syncthreads();   // plane ic published; b' of plane ic-1 complete
-      if (ic > i0) solve_store(ic - 1);
     }
-    __syncthreads();     // b' of the last plane complete
-    solve_store(i1 - 1);
     return;
   }
   // ---- stencil waves: thread = stored position
@@ -937,7 +987,8 @@ __global__ void __launch_bounds__(512)
         double r = USE_RU ? c.ro * y0.ry : 0.0;
         double rhs = (c.ga * h + r + c.alre * dxxvy - dyp) * c.dt;
         if (keep_ru) sth<H32>(ruy, C + sb, iny ? h : 0.0);
-        bqc[1 * ldb + s] = rhs * cfv(FC_YSC) + ((cfv(FC_YDL) * vym + vyc) + cfv(FC_YDU) * vyp);
+        // row nx is outside the nxm system: b' = 0 there, as the solve expects
+        bqc[1 * ldb + s] = iny ? rhs * cfv(FC_YSC) + ((cfv(FC_YDL) * vym + vyc) + cfv(FC_YDU) * vyp) : 0.0;
       }
       // ---- vz (ExplicitTermsVZ.F90:56-101, ImplicitAndUpdateVZ.F90:48-64)
       {
@@ -954,7 +1005,7 @@ __global__ void __launch_bounds__(512)
         double r = USE_RU ? c.ro * y0.rz : 0.0;
         double rhs = (c.ga * h + r + c.alre * dxxvz - dzp) * c.dt;
         if (keep_ru) sth<H32>(ruz, C + sb, iny ? h : 0.0);
-        bqc[2 * ldb + s] = rhs * cfv(FC_YSC) + ((cfv(FC_YDL) * vzm + vzc) + cfv(FC_YDU) * vzp);
+        bqc[2 * ldb + s] = iny ? rhs * cfv(FC_YSC) + ((cfv(FC_YDL) * vzm + vzc) + cfv(FC_YDU) * vzp) : 0.0;
       }
       // ---- temperature (ExplicitTermsTemp.F90:52-93, ImplicitAndUpdateTemp.F90:41-54)
       {
@@ -1338,8 +1389,6 @@ hipError_t launch_stencil_rhs(const FusedArgs &a, hipStream_t st) {
 #undef SR1
   return hipErrorInvalidValue;
 }
-
-static int fu_pick_lm(int L) { return L <= 8 ? 8 : (L <= 16 ? 16 : (L <= 25 ? 25 : (L <= 28 ? 28 : -1))); }
 
 bool substep_update_fits(const Geom &g) {
   return g.ldx <= FU_MAX_LDX && fu_pick_lm(g.L) > 0 && substep_update_lds(g) <= 160 * 1024 &&
