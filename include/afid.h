@@ -164,10 +164,11 @@ int afid_steps(afid_ctx *ctx, int ntime0, int nsteps, int idtv, double dt, doubl
                double limitcfl, double *dt_last, int *errorcode, int *steps_done);
 /* CalcMaxCFL (CalcMaxCFL.F90:11-48, with its 1e-8 floor) of the fields the
  * last device step left, as the device step control computed it for the next
- * step: in FP64 the step graph folds the reduction into its last pressure
- * correction, so the fields are not read again.  Equals afid_calc_max_cfl of
- * the same fields.  AFID_ERR_STATE before any device step and in the mixed
- * mode, whose step graph reduces at the start of each step instead. */
+ * step when the context was created with AFID_CFL=fold in the environment
+ * (FP64 only): the step graph then folds the reduction into its last pressure
+ * correction instead of reducing the fields at the start of each step.
+ * Equals afid_calc_max_cfl of the same fields.  AFID_ERR_STATE before any
+ * device step and without the fold. */
 int afid_steps_cfl(afid_ctx *ctx, double *cflm);
 
 /* ---- concurrent cells (batched sweeps, serving) --------------------------------

@@ -262,14 +262,35 @@ def test_conduction_fixed_point_on_gpu(afid):
     cell.close()
 
 
+def _cell_with_env(afid, params, **env):
+    import os
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    try:
+        return afid.Cell(params)
+    finally:
+        for k, v in old.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
+
+
 @pytest.mark.parametrize("c", SMALL_CASES, ids=case_id)
 def test_folded_cfl_equals_reduction(afid, c):
-    """The FP64 step graph takes CalcMaxCFL from its last pressure correction
-    instead of a separate pass over the fields: after every step the value it
-    carries equals CalcMaxCFL of the stored fields bit for bit, so the dt
-    sequence is the one a separate reduction gives.  Also for an upload
-    between calls, after which the next call starts from a fresh reduction."""
-    cell = afid.Cell(afid.CellParams(nsst=3, alx3=1.0, **c))
+    """With AFID_CFL=fold the FP64 step graph takes CalcMaxCFL from its last
+    pressure correction instead of a separate pass over the fields: after
+    every step the value it carries equals CalcMaxCFL of the stored fields bit
+    for bit, so the dt sequence is the one a separate reduction gives.  Also
+    for an upload between calls, after which the next call starts from a
+    fresh reduction.  Without the fold there is no carried value."""
+    plain = afid.Cell(afid.CellParams(nsst=3, alx3=1.0, **c))
+    plain.init_linear_noise(6, 1e-3)
+    plain.steps(0, 2, 1, 2e-3, 1e-4, 5e-3, 1.2)
+    with pytest.raises(afid.AfidError):
+        plain.steps_cfl()
+    plain.close()
+    cell = _cell_with_env(afid, afid.CellParams(nsst=3, alx3=1.0, **c), AFID_CFL="fold")
     cell.init_linear_noise(6, 1e-3)
     for n in range(5):
         err, _, done = cell.steps(n, 1 if n < 2 else 2, 1, 2e-3, 1e-4, 5e-3, 1.2)
@@ -309,8 +330,9 @@ def test_device_steps_match_host_steps(afid, c):
     dev.close()
 
 
+@pytest.mark.parametrize("fold", [False, True], ids=["reduce", "fold"])
 @pytest.mark.parametrize("c", SMALL_CASES, ids=case_id)
-def test_replayed_step_graphs_take_each_steps_cfl(afid, c):
+def test_replayed_step_graphs_take_each_steps_cfl(afid, c, fold):
     """Many steps from one afid_steps call (the step graph replayed back to
     back, each step's dt from the CFL number the previous step's last
     correction folded on the device) against host-driven steps that reduce
@@ -319,7 +341,7 @@ def test_replayed_step_graphs_take_each_steps_cfl(afid, c):
     the time and the fields."""
     ctl = (1, 2e-3, 1e-6, 5e-3, 0.005)
     host = afid.Cell(afid.CellParams(nsst=3, alx3=1.0, **c))
-    dev = afid.Cell(afid.CellParams(nsst=3, alx3=1.0, **c))
+    dev = _cell_with_env(afid, afid.CellParams(nsst=3, alx3=1.0, **c), AFID_CFL="fold" if fold else "reduce")
     rng = np.random.default_rng(11)
     nzm, nym, nx = c["nzm"], c["nym"], c["nxm"] + 1
     vel = {}
@@ -339,7 +361,8 @@ def test_replayed_step_graphs_take_each_steps_cfl(afid, c):
     assert (e, done) == (0, 14)
     assert dt == pytest.approx(dts[-1], rel=1e-12)
     assert dev.time == pytest.approx(host.time, rel=1e-12)
-    assert dev.steps_cfl() == dev.CalcMaxCFL()
+    if fold:
+        assert dev.steps_cfl() == dev.CalcMaxCFL()
     for nm in ("vx", "vy", "vz", "temp"):
         assert rel_l2(dev.download(nm), host.download(nm)) < 1e-13, nm
     host.close()
