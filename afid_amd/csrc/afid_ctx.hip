@@ -86,7 +86,9 @@ struct afid_ctx {
   int zm = 0;                     // z planes per stencil block chosen by afid_tune (0: default)
   bool fused_update = false;      // substep update in one pass (k_substep_update) rather than two
   bool cfl_fold = false;          // device steps take CalcMaxCFL from the last correction (FP64 only)
-  DevPlan dpy, dpz;
+  DevPlan dpy, dpz, dpzq;
+  FFTPlan pzq{};                  // Q = nzm/16-point plan of the matrix-core z transform
+  bool zfft_mfma = false;         // AFID_ZFFT=mfma: z transforms with a 16-point DFT stage on MFMA
   RealPlan py{};
   FFTPlan pz{};
   // natural-order device array views
@@ -534,6 +536,8 @@ PoissonArgs poisson_args(afid_ctx *c, int from_velocity, int ns = 0, bool dev = 
   a.sc = scalars(c);
   a.py = c->py;
   a.pz = c->pz;
+  a.pzq = c->pzq;
+  a.zmf = c->zfft_mfma ? 1 : 0;
   a.from_velocity = from_velocity;
   a.spec32 = c->mixed;
   return a;
@@ -630,7 +634,8 @@ int afid_destroy(afid_ctx *c) {
   hipFree(c->slab_buf);
   for (int q = 0; q < 4; q++) hipFree(c->fac_sub[q]);
   void *ps[] = {c->stale, c->kc, c->fac_x, c->S, c->piv, c->nat, c->partial, c->out, c->stage,
-                c->dpy.tw, c->dpy.rev, c->dpy.post, c->dpz.tw, c->dpz.rev, c->dpz.post};
+                c->dpy.tw, c->dpy.rev, c->dpy.post, c->dpz.tw, c->dpz.rev, c->dpz.post,
+                c->dpzq.tw, c->dpzq.rev, c->dpzq.post};
   for (void *q : ps) hipFree(q);
   if (c->pin_fac) hipHostFree(c->pin_fac);
   if (c->fac_ev) hipEventDestroy(c->fac_ev);
@@ -713,6 +718,13 @@ int afid_create(afid_ctx **out, const afid_params *p, const afid_metrics *m) {
     TRY(make_plan(c, c->py.H, &c->py.c, &c->dpy, N));
     c->py.post = c->dpy.post;
     TRY(make_plan(c, g.nzm, &c->pz, &c->dpz, 0));
+    {   // AFID_ZFFT=mfma: the z transforms' 16-point stage on the matrix cores (FP64)
+      const char *zf = std::getenv("AFID_ZFFT");
+      if (zf && std::strcmp(zf, "mfma") == 0 && p->precision == AFID_PREC_FP64 && zfft_mfma_fits(g.nzm)) {
+        TRY(make_plan(c, g.nzm / 16, &c->pzq, &c->dpzq, 0));
+        c->zfft_mfma = true;
+      }
+    }
     if (c->py.H > 1280 || g.nzm > 2560) TRY(AFID_ERR_ARG);
     // the dynamic-LDS limits are per kernel and process-wide: only ever raised,
     // so that cells created earlier keep what they launch with

@@ -330,6 +330,158 @@ __global__ void __launch_bounds__(256) k_zfft(PoissonArgs a, R sign) {
   }
 }
 
+// ---------------------------------------------------------------- z transform with a matrix-core stage
+// N = 16 Q.  With n = Q a + b (a < 16, b < Q) and k = c + 16 d:
+//   Y[c + 16 d] = sum_b W_Q^{bd} W_N^{bc} sum_a W_16^{ac} X[Q a + b],
+// so the 16-point DFTs over a are one complex matrix product per tile of 16
+// rows (q, b): Z = X F16, with X[(q,b)][a] and F16[a][c] = W_16^{ac}, done as
+// four real products per K step of 4 on v_mfma_f64_16x16x4f64.  Lane l holds
+// A[l % 16][l / 16] and B[l / 16][l % 16], and register r of D[(l / 16) + 4r]
+// [l % 16] (checked on the device).  The twiddles W_N^{bc} follow, then the
+// Q-point transforms over b for the 16 c of each line run in LDS with the
+// radix stages of lds_fft_wave.  A wave holds the products of its TPW tiles
+// in registers until every wave has read the input, then writes them in place.
+typedef double dbl4_t __attribute__((ext_vector_type(4)));
+
+// In-place DIT over nl lines of length pl.n stored contiguously from `lines`,
+// input in digit-reversed order, by one wavefront (as lds_fft_wave, for any
+// number of lines).
+__device__ __forceinline__ void lds_fft_lines(double2 *lines, int nl, const FFTPlan &pl, double sign, int lane) {
+  const int n = pl.n;
+  const bool sw = (n & 7) == 0;
+  const double2 *tw = pl.tw;
+  int m = 1;
+  for (int st = 0; st < pl.nf; st++) {
+    const int r = pl.f[st];
+    const int M = m * r;
+    const int nb = n / r;
+    const int tws = n / M;
+    const unsigned mdiv = pl.mdiv[st];
+    const int total = nl * nb;
+    for (int wid = lane; wid < total; wid += 64) {
+      const int q = wid / nb, w = wid - q * nb;
+      const int bb = m == 1 ? w : (int)__umulhi((unsigned)w, mdiv);
+      const int u = w - bb * m;
+      double2 *line = lines + q * n;
+      const int e0 = bb * M + u;
+      const int t1i = u * tws;
+      const double2 t1 = twid(tw, t1i, sign), t2 = twid(tw, min(2 * t1i, n - 1), sign);
+      const double2 t3 = twid(tw, min(3 * t1i, n - 1), sign), t4 = twid(tw, min(4 * t1i, n - 1), sign);
+#define X(v) line[swz(e0 + (v) * m, sw)]
+      if (r == 2) {
+        double2 x0 = X(0), x1 = cmul(X(1), t1);
+        X(0) = cadd(x0, x1);
+        X(1) = csub(x0, x1);
+      } else if (r == 4) {
+        double2 x0 = X(0), x1 = cmul(X(1), t1), x2 = cmul(X(2), t2), x3 = cmul(X(3), t3);
+        double2 a0 = cadd(x0, x2), a1 = csub(x0, x2);
+        double2 b0 = cadd(x1, x3), b1 = mul_si(csub(x1, x3), sign);
+        X(0) = cadd(a0, b0);
+        X(1) = cadd(a1, b1);
+        X(2) = csub(a0, b0);
+        X(3) = csub(a1, b1);
+      } else if (r == 3) {
+        const double s3 = 0.86602540378443864676372317075294;  // sin(2 pi/3)
+        double2 x0 = X(0), x1 = cmul(X(1), t1), x2 = cmul(X(2), t2);
+        double2 p1 = cadd(x1, x2), p2 = mul_si(csub(x1, x2), sign * s3);
+        double2 q0 = mkc(x0.x - 0.5 * p1.x, x0.y - 0.5 * p1.y);
+        X(0) = cadd(x0, p1);
+        X(1) = cadd(q0, p2);
+        X(2) = csub(q0, p2);
+      } else {  // r == 5
+        const double c1 = 0.30901699437494742410229341718282, c2 = -0.80901699437494742410229341718282;
+        const double s1 = 0.95105651629515357211643933337938, s2 = 0.58778525229247312916870595463907;
+        double2 x0 = X(0), x1 = cmul(X(1), t1), x2 = cmul(X(2), t2), x3 = cmul(X(3), t3), x4 = cmul(X(4), t4);
+        double2 p1 = cadd(x1, x4), p2 = cadd(x2, x3), p3 = csub(x1, x4), p4 = csub(x2, x3);
+        double2 a1 = mkc(x0.x + c1 * p1.x + c2 * p2.x, x0.y + c1 * p1.y + c2 * p2.y);
+        double2 a2 = mkc(x0.x + c2 * p1.x + c1 * p2.x, x0.y + c2 * p1.y + c1 * p2.y);
+        double2 b1 = mul_si(mkc(s1 * p3.x + s2 * p4.x, s1 * p3.y + s2 * p4.y), sign);
+        double2 b2 = mul_si(mkc(s2 * p3.x - s1 * p4.x, s2 * p3.y - s1 * p4.y), sign);
+        X(0) = cadd(x0, cadd(p1, p2));
+        X(1) = cadd(a1, b1);
+        X(2) = cadd(a2, b2);
+        X(3) = csub(a2, b2);
+        X(4) = csub(a1, b1);
+      }
+#undef X
+    }
+    wave_lds_sync();
+    m = M;
+  }
+}
+
+template <int TPW>
+__global__ void __launch_bounds__(256) k_zfft_mfma(PoissonArgs a, double sign) {
+  extern __shared__ double2 lds[];
+  const int N = a.pz.n, Q = 16 * TPW;
+  const int ntile = (a.nymh + TM - 1) / TM;
+  const int64_t tile = fft_tile((int64_t)ntile * a.g.nxm);
+  if (tile < 0 || (a.ds && coh_int(&a.ds->halted))) return;
+  const int m0 = (int)(tile % ntile) * TM, k = (int)(tile / ntile);
+  double2 *S = a.S + (int64_t)k * a.g.nzm * a.ldm;
+  const int tot = TM * N;
+  for (int base = threadIdx.x; base < tot; base += 256 * ZLB) {   // natural order: line q at lds[q N + n]
+    double2 v[ZLB];
+#pragma unroll
+    for (int b = 0; b < ZLB; b++) {
+      const int ic = min(base + 256 * b, tot - 1);
+      const int q = ic % TM, nn = ic / TM;
+      v[b] = S[(int64_t)nn * a.ldm + min(m0 + q, a.nymh - 1)];
+    }
+#pragma unroll
+    for (int b = 0; b < ZLB; b++) {
+      const int idx = base + 256 * b;
+      if (idx >= tot) continue;
+      const int q = idx % TM, nn = idx / TM;
+      lds[q * N + nn] = m0 + q < a.nymh ? v[b] : mkc(0.0, 0.0);
+    }
+  }
+  __syncthreads();
+  const int lane = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
+  const int i = lane & 15, kq = lane >> 4, cc = lane & 15;
+  const double2 *tw = a.pz.tw;
+  dbl4_t dre[TPW], dim[TPW];
+#pragma unroll
+  for (int t = 0; t < TPW; t++) {
+    const int rho = (w * TPW + t) * 16 + i;   // row (q, b) of the product
+    const int q = rho / Q, b = rho - q * Q;
+    dre[t] = dbl4_t{0.0, 0.0, 0.0, 0.0};
+    dim[t] = dbl4_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int kk = 0; kk < 4; kk++) {
+      const int aa = 4 * kk + kq;
+      const double2 x = lds[q * N + Q * aa + b];
+      const double2 f = twid(tw, Q * ((aa * cc) & 15), sign);   // W_16^{a c} = W_N^{Q a c}
+      dre[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, f.x, dre[t], 0, 0, 0);
+      dre[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-x.y, f.y, dre[t], 0, 0, 0);
+      dim[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, f.y, dim[t], 0, 0, 0);
+      dim[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.y, f.x, dim[t], 0, 0, 0);
+    }
+  }
+  __syncthreads();   // every wave has read its input rows
+  const bool sw = (Q & 7) == 0;
+#pragma unroll
+  for (int t = 0; t < TPW; t++) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int rho = (w * TPW + t) * 16 + 4 * r + kq;
+      const int q = rho / Q, b = rho - q * Q;
+      const double2 z = cmul(mkc(dre[t][r], dim[t][r]), twid(tw, b * cc, sign));
+      lds[q * N + cc * Q + swz(a.pzq.rev[b], sw)] = z;   // line (q, c), digit-reversed for the DIT
+    }
+  }
+  __syncthreads();
+  lds_fft_lines(lds + w * N, 16, a.pzq, sign, lane);   // wave w: the 16 lines of q = w
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
+    const int q = idx % TM, nn = idx / TM;
+    const int c = nn & 15, d = nn >> 4;
+    if (m0 + q < a.nymh) S[(int64_t)nn * a.ldm + m0 + q] = lds[q * N + c * Q + swz(d, sw)];
+  }
+}
+
+bool zfft_mfma_fits(int nzm) { return nzm == 256 || nzm == 512 || nzm == 768 || nzm == 1024; }
+
 // ---------------------------------------------------------------- x solve
 // Row k of the (m,n) mode's system, normalised to a unit diagonal
 // (SolvePressureCorrection.F90:117-141):
@@ -627,7 +779,14 @@ static hipError_t fft_configure_lds_r(size_t bytes) {
 }
 hipError_t fft_configure_lds(size_t bytes) {
   hipError_t e = fft_configure_lds_r<double>(bytes);
-  return e != hipSuccess ? e : fft_configure_lds_r<float>(bytes);
+  if (e != hipSuccess) return e;
+  e = fft_configure_lds_r<float>(bytes);
+  if (e != hipSuccess) return e;
+  const void *mk[] = {(const void *)k_zfft_mfma<1>, (const void *)k_zfft_mfma<2>, (const void *)k_zfft_mfma<3>,
+                      (const void *)k_zfft_mfma<4>};
+  for (const void *f : mk)
+    if ((e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes)) != hipSuccess) return e;
+  return hipSuccess;
 }
 
 static dim3 tile_grid(int64_t total) { return dim3((unsigned)(((total + 7) / 8) * 8)); }
@@ -643,6 +802,15 @@ hipError_t launch_yfft_forward(const PoissonArgs &a, hipStream_t st) {
 hipError_t launch_zfft(const PoissonArgs &a, int sign, hipStream_t st) {
   dim3 grid = tile_grid((int64_t)((a.nymh + TM - 1) / TM) * a.g.nxm);
   size_t shm = cplx_bytes(a) * (size_t)TM * a.pz.n;
+  if (a.zmf && !a.spec32) {
+    switch (a.pz.n / 256) {
+      case 1: hipLaunchKernelGGL(k_zfft_mfma<1>, grid, dim3(256), shm, st, a, (double)sign); break;
+      case 2: hipLaunchKernelGGL(k_zfft_mfma<2>, grid, dim3(256), shm, st, a, (double)sign); break;
+      case 3: hipLaunchKernelGGL(k_zfft_mfma<3>, grid, dim3(256), shm, st, a, (double)sign); break;
+      default: hipLaunchKernelGGL(k_zfft_mfma<4>, grid, dim3(256), shm, st, a, (double)sign); break;
+    }
+    return hipGetLastError();
+  }
   if (a.spec32) hipLaunchKernelGGL(k_zfft<float>, grid, dim3(256), shm, st, a, (float)sign);
   else hipLaunchKernelGGL(k_zfft<double>, grid, dim3(256), shm, st, a, (double)sign);
   return hipGetLastError();

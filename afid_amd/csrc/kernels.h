@@ -91,6 +91,8 @@ struct PoissonArgs {
   StepScalars sc;
   RealPlan py;
   FFTPlan pz;
+  FFTPlan pzq;           // z transform on the matrix cores (zmf): the Q = nzm/16-point plan
+  int zmf;               // 1: z transforms with a 16-point DFT stage on the matrix cores
   int from_velocity;     // 1: y-FFT input is div(u)/(dt al) computed on the fly; 0: dph
   const DevStep *ds;
   int ns;
@@ -114,6 +116,7 @@ hipError_t column_configure_lds(size_t bytes);
 
 hipError_t launch_yfft_forward(const PoissonArgs &a, hipStream_t st);
 hipError_t launch_zfft(const PoissonArgs &a, int sign, hipStream_t st);
+bool zfft_mfma_fits(int nzm);   // nzm = 16 Q with Q in {16, 32, 48, 64} (radices 2..5)
 hipError_t launch_xsolve_pressure(const PoissonArgs &a, hipStream_t st);
 hipError_t launch_xsolve_pivots(const PoissonArgs &a, double *piv, hipStream_t st);
 void xsolve_shape(int nxm, int *np, int *lm);   // lanes per mode, register rows per lane
