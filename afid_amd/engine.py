@@ -520,6 +520,15 @@ class Cell:
         self._check(self.L.afid_check_divergence(self.h, ctypes.byref(v)), "CheckDivergence")
         return v.value
 
+    def max_divergence(self) -> float:
+        """CheckDivergence when every field is finite, else inf.  The maximum in
+        CheckDivergence (CheckDivergence.F90:27-40) skips NaN as the
+        reference's comparison does, so a cell that blew up would report a small
+        value; GlobalQuantities' sums carry any NaN or inf through."""
+        if not np.all(np.isfinite(self.GlobalQuantities())):
+            return float("inf")
+        return self.CheckDivergence()
+
     def CalcPlateNu(self):
         a, b = ctypes.c_double(), ctypes.c_double()
         self._check(self.L.afid_calc_plate_nu(self.h, ctypes.byref(a), ctypes.byref(b)), "CalcPlateNu")

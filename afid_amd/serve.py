@@ -95,7 +95,7 @@ class Server:
     """Continuous-batching request loop over resident cells.
 
     make_cell(params) -> object with tune, init_linear_noise, steps_async,
-    steps_more, busy, steps_wait, CheckDivergence, close (engine.Cell by default);
+    steps_more, busy, steps_wait, max_divergence, close (engine.Cell by default);
     estimate(params) -> bytes; on_finish(request, cell) runs before the cell is
     closed (the tests read the fields there)."""
 
@@ -174,7 +174,7 @@ class Server:
                         continue
                     # all steps ran, or the step control halted with an error code
                     # (main.F90 stops there): the request ends now
-                    div = a.cell.CheckDivergence()
+                    div = a.cell.max_divergence()   # inf if a field is not finite
                     if self.on_finish:
                         self.on_finish(a.req, a.cell)
                     fin = Completion(rid, a.req.arrival, a.admitted, self.clock() - t0, ndone, err, div,

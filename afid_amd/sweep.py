@@ -104,7 +104,7 @@ def run_batched(specs: List[CellSpec], on_done: Optional[Callable[[int, Cell], N
             err, _, done = c.steps_wait()
             res.errorcodes.append(err)
             res.steps_done.append(done)
-            res.max_divergence = max(res.max_divergence, c.CheckDivergence())
+            res.max_divergence = max(res.max_divergence, c.max_divergence())
             if on_done:
                 on_done(q, c)
     finally:
@@ -138,7 +138,7 @@ def run_serial(specs: List[CellSpec], host_driven: bool = True) -> SweepResult:
                                        ctl["limitcfl"])
             res.errorcodes.append(err)
             res.steps_done.append(done)
-            res.max_divergence = max(res.max_divergence, c.CheckDivergence())
+            res.max_divergence = max(res.max_divergence, c.max_divergence())
         finally:
             c.close()
     res.seconds = time.perf_counter() - t0

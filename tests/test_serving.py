@@ -94,7 +94,7 @@ class FakeCell:
             return 166, 1e-3, min(self.queued, 5)
         return 0, 1e-3, self.queued
 
-    def CheckDivergence(self):
+    def max_divergence(self):
         return 1e-15
 
     def close(self):
