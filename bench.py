@@ -27,11 +27,14 @@ HBM_PEAK_GBS = 8000.0
 # Algorithmic FP64 words per cell of each kernel: the distinct fields it must
 # read and write once (8 B each).  stencil_rhs reads vx,vy,vz,T,pr and the four
 # histories and writes the histories and the four right-hand sides b + A q_old;
-# column_solve reads those and writes the four new fields; fused_update is the
-# two in one pass.  Spectral arrays hold nym/2+1 complex values per y line, one
-# word per cell.  The substep figure is the reference's per-routine compulsory
-# traffic: explicit VX 5, VY 4, VZ 4, Temp 5; implicit VX/VY/VZ 6 each, Temp 5;
-# divergence 4; pressure solve 10 (5 passes); CorrectVelocity 7; CorrectPressure 3.
+# column_solve reads those and writes the four new fields (u* in place);
+# substep_update (AFID_UPDATE=fused) is the two in one pass, writing u* and the
+# new temperature apart from the fields.  Spectral arrays hold nym/2+1 complex
+# values per y line, one word per cell.  The correction reads u*, dph and pr and
+# writes the velocities and pr in place.  The substep figure is the reference's
+# per-routine compulsory traffic: explicit VX 5, VY 4, VZ 4, Temp 5; implicit
+# VX/VY/VZ 6 each, Temp 5; divergence 4; pressure solve 10 (5 passes);
+# CorrectVelocity 7; CorrectPressure 3.
 WORDS = {
     # RK3: the first substep (ro = 0) does not read the four histories (13
     # words); the third does not store them, since the next step's first
@@ -39,7 +42,7 @@ WORDS = {
     # HIP-event average covers all three.
     "stencil_rhs": (13 + 17 + 13) / 3,
     "column_solve": 4 + 4,
-    "fused_update": 9 + 4 + 4,
+    "substep_update": (13 + 17 + 13) / 3,
     "div_yfft_r2c": 3 + 1,
     "zfft_fwd": 2,
     "xsolve_pressure": 2,
@@ -58,11 +61,11 @@ PMC_KERNELS = {
     # graph steps run the device-scalar instances
     "stencil_rhs": [("k_stencil_rhs<u32, F, T, F>", 1 / 3), ("k_stencil_rhs<u32, T, T, F>", 2 / 3)],
     "column_solve": [("k_column_solve<25, u32>", 1.0)],
-    "correct_vel_pr": [("k_correct", 1.0)],
+    "correct_vel_pr": [("k_correct<F>", 1.0)],
     "div_yfft_r2c": [("k_yfft_forward<double>", 1.0)],
     "zfft_fwd": [("k_zfft<double>", 1.0)],
     "zfft_bwd": [("k_zfft<double>", 1.0)],
-    "xsolve_pressure": [("k_xsolve_pressure<double>", 1.0)],
+    "xsolve_pressure": [("k_xsolve_pressure<32, 12, double>", 1.0)],
     "yfft_c2r": [("k_yfft_inverse<double>", 1.0)],
 }
 
