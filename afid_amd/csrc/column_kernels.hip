@@ -336,7 +336,11 @@ __global__ void __launch_bounds__(SY_WG)
   const int sraw = q * 64 + lane;
   const int s = min(sraw, ldx - 1);
   const int k = k_of_pos(s, L);
-  const bool live = colok && sraw < ldx && k < nx;
+  // Every stored position of a column is written, the padding (k >= nx) as
+  // zeros: a 128-byte line written in part costs the memory system about as
+  // much again as a whole one (1.6 ms of 8.6 per launch at 384x768x768).
+  const bool stv = colok && sraw < ldx;
+  const bool pad = k >= nx;
   const bool jp_in = w + 1 < ncols, jm_in = w > 0;        // neighbour column in this block
   const int jm = wrapm(jc - 1, g.nym), jp = wrapp(jc + 1, g.nym);
   const OffT cb = (OffT)ldx * sizeof(double);
@@ -404,7 +408,7 @@ __global__ void __launch_bounds__(SY_WG)
     const double vzm = cur.vzm, vzc = cur.vzc, vzp = cur.vzp, tm = cur.tm, tc = cur.tc, tp = cur.tp;
     const double prm = cur.prm, prc = cur.prc;
     const double vxip = nxt.vxc, vyip = nxt.vyc, vzip = nxt.vzc, vzipm = nxt.vzm, tip = nxt.tc;
-    if (live) {
+    if (stv) {
       // ---- vx (ExplicitTermsVX.F90:55-101, ImplicitAndUpdateVX.F90:47-60)
       {
         double hxz = (((vzip + vzipm) * (vxip + vxc)) - ((vzc + vzm) * (vxc + vxim))) * c.udz4;
@@ -420,8 +424,8 @@ __global__ void __launch_bounds__(SY_WG)
         double rhs = (c.ga * q + r + c.alre * dxxvx - dxp) * c.dt;
         double vin = rhs * cfv(SC_XSC) + ((cfv(SC_XDL) * vxm + vxc) + cfv(SC_XDU) * vxp);
         double vbd = stale0 + vxc;
-        if (keep_ru) sth<H32>(rux, C + sb, inx ? q : 0.0);
-        stf(bx, C + sb, inx ? vin : vbd);
+        if (keep_ru) sth<H32>(rux, C + sb, pad ? 0.0 : (inx ? q : 0.0));
+        stf(bx, C + sb, pad ? 0.0 : (inx ? vin : vbd));
       }
       // ---- vy (ExplicitTermsVY.F90:52-98, ImplicitAndUpdateVY.F90:52-68); kmv/kpv clamps
       {
@@ -438,8 +442,8 @@ __global__ void __launch_bounds__(SY_WG)
         double r = USE_RU ? c.ro * ryo : 0.0;
         double rhs = (c.ga * h + r + c.alre * dxxvy - dyp) * c.dt;
         double vin = rhs * cfv(SC_YSC) + ((cfv(SC_YDL) * vym + vyc) + cfv(SC_YDU) * vyp);
-        if (keep_ru) sth<H32>(ruy, C + sb, iny ? h : 0.0);
-        stf(by, C + sb, vin);   // row nx-1 is outside the nxm system and never read
+        if (keep_ru) sth<H32>(ruy, C + sb, pad ? 0.0 : (iny ? h : 0.0));
+        stf(by, C + sb, pad ? 0.0 : vin);   // row nx-1 is outside the nxm system and never read
       }
       // ---- vz (ExplicitTermsVZ.F90:56-101, ImplicitAndUpdateVZ.F90:48-64)
       {
@@ -456,8 +460,8 @@ __global__ void __launch_bounds__(SY_WG)
         double r = USE_RU ? c.ro * rzo : 0.0;
         double rhs = (c.ga * h + r + c.alre * dxxvz - dzp) * c.dt;
         double vin = rhs * cfv(SC_YSC) + ((cfv(SC_YDL) * vzm + vzc) + cfv(SC_YDU) * vzp);
-        if (keep_ru) sth<H32>(ruz, C + sb, iny ? h : 0.0);
-        stf(bz, C + sb, vin);
+        if (keep_ru) sth<H32>(ruz, C + sb, pad ? 0.0 : (iny ? h : 0.0));
+        stf(bz, C + sb, pad ? 0.0 : vin);
       }
       // ---- temperature (ExplicitTermsTemp.F90:52-93, ImplicitAndUpdateTemp.F90:41-54)
       {
@@ -471,8 +475,8 @@ __global__ void __launch_bounds__(SY_WG)
         double r = USE_RU ? c.ro * rto : 0.0;
         double rhs = (c.ga * h + r + c.alpec * dxxt) * c.dt;
         double vin = rhs * cfv(SC_TSC) + ((cfv(SC_TDL) * tm + tc) + cfv(SC_TDU) * tp);
-        if (keep_ru) sth<H32>(rut, C + sb, inx ? h : 0.0);
-        stf(bt, C + sb, inx ? vin : tc);   // row 1 gets the vz increment added by the solve pass
+        if (keep_ru) sth<H32>(rut, C + sb, pad ? 0.0 : (inx ? h : 0.0));
+        stf(bt, C + sb, pad ? 0.0 : (inx ? vin : tc));   // row 1 gets the vz increment added by the solve pass
       }
     }
     vxim = vxc; vximp = vxp; vyim = vyc; vzim = vzc; tim = tc; prim = prc;
@@ -557,6 +561,11 @@ __global__ void __launch_bounds__(SOLVE_WG)
   const int cntm = live ? max(0, min(L, nxm - k0)) : 0;
   const double *rX = rows + (SYS_X * RW_N) * ldx, *rY = rows + (SYS_YZ * RW_N) * ldx,
                *rT = rows + (SYS_T * RW_N) * ldx;
+  // Row nx-1 of vy and vz is outside the nxm systems and keeps the field's
+  // value; it is stored again with the rest, so that every 128-byte line of a
+  // column is written whole (one broadcast load per column).
+  const OffT Ctop = C + (OffT)pos_of_k(nx - 1, L) * sizeof(double);
+  const double vytop = ldf(a.f.vy, Ctop), vztop = ldf(vz, Ctop);
   double b[LM];
   load_chunk<LM, OffT>(b, bx, C, cntx, L, p);
   {  // vx
@@ -579,8 +588,7 @@ __global__ void __launch_bounds__(SOLVE_WG)
       for (int t = 0; t < LM; t++)
         if (t < L) {
           const int k = k0 + t, s = t * P + p;
-          // row nx is outside the nxm system: nothing reads it from here
-          if (k != nx - 1) stf(oy, C + (OffT)s * sizeof(double), k < nxm ? b[t] + fh[s] * cin : 0.0);
+          stf(oy, C + (OffT)s * sizeof(double), k == nx - 1 ? vytop : (k < nxm ? b[t] + fh[s] * cin : 0.0));
         }
   }
   load_chunk<LM, OffT>(b, bz, C, cntm, L, p);
@@ -595,7 +603,7 @@ __global__ void __launch_bounds__(SOLVE_WG)
       for (int t = 0; t < LM; t++)
         if (t < L) {
           const int k = k0 + t, s = t * P + p;
-          if (k != nx - 1) stf(oz, C + (OffT)s * sizeof(double), k < nxm ? b[t] + fh[s] * cin : 0.0);
+          stf(oz, C + (OffT)s * sizeof(double), k == nx - 1 ? vztop : (k < nxm ? b[t] + fh[s] * cin : 0.0));
         }
   }
   dvz0 = __shfl(dvz0, 0, P);
@@ -872,8 +880,9 @@ __global__ void __launch_bounds__(512)
               v = k == 0 ? a.sc.tempbp : (k == nx - 1 ? a.sc.temptp : b[t]);
               v = k < nx ? v : 0.0;
             } else {
-              if (k == nx - 1) continue;   // row nx is outside the nxm system: never written
-              v = k < nxm ? b[t] : 0.0;
+              // row nx is outside the nxm system and keeps the field's value,
+              // which u* carries to the correction
+              v = k == nx - 1 ? ldf(grp == 1 ? vy : vz, C + (OffT)sl * sizeof(double)) : (k < nxm ? b[t] : 0.0);
             }
             stf(outp, C + (OffT)sl * sizeof(double), v);
           }
@@ -1226,7 +1235,10 @@ __global__ void __launch_bounds__(SY_WG) k_correct(CorrectArgs a, int which, int
   const int s = min(sraw, ldx - 1);
   const int k = k_of_pos(s, L);
   const bool live = colok && sraw < ldx && k <= nxm - 1;
-  const bool top = colok && sraw < ldx && k == g.nx - 1;   // vx wall row: u* (0) passes to the field
+  // Positions k >= nxm (the wall row of vx, where u* is 0, row nx-1 of the
+  // other fields, and the padding) keep what u* and pr hold there; they are
+  // stored too, so that every 128-byte line of a column is written whole.
+  const bool stv = colok && sraw < ldx;
   const bool jp_in = w + 1 < ncols, jm_in = w > 0;
   const int jm = wrapm(jc - 1, g.nym), jp = wrapp(jc + 1, g.nym);
   const int skm = (k == 0 || k > nxm - 1) ? s : pos_of_k(k - 1, L);
@@ -1277,15 +1289,15 @@ __global__ void __launch_bounds__(SY_WG) k_correct(CorrectArgs a, int which, int
     if (jp_in) djp = buf[tid + 64];
     if (CFL && jp_in) uyjp = buf[SY_WG + tid + 64];
     par ^= 1;
-    if (live) {
+    if (stv) {
       if (which & 1) {
         const double nvx = corr(vxo, d - dkm, usukm);
         const double nvy = corr(vyo, d - djm, udy);
         const double nvz = corr(vzo, d - dim_, udz);
-        qx[C + s] = nvx;
-        qy[C + s] = nvy;
-        qz[C + s] = nvz;
-        if (CFL) {
+        qx[C + s] = live ? nvx : vxo;
+        qy[C + s] = live ? nvy : vyo;
+        qz[C + s] = live ? nvz : vzo;
+        if (CFL && live) {
           const double nvxk1 = k + 1 <= nxm - 1 ? corr(uxk1, dkp - d, usukp) : uxk1;
           const double nvyjp = corr(uyjp, djp - d, udy);
           const double nvzip = corr(uzip, dip - d, udz);
@@ -1295,10 +1307,10 @@ __global__ void __launch_bounds__(SY_WG) k_correct(CorrectArgs a, int which, int
         }
       }
       if (which & 2)
-        qp[C + s] = pro + d - be * ((dip - 2.0 * d + dim_) * c.dzq + (djp - 2.0 * d + djm) * c.dyq +
-                                    (dkp * app + d * acc + dkm * amm));
+        qp[C + s] = live ? pro + d - be * ((dip - 2.0 * d + dim_) * c.dzq + (djp - 2.0 * d + djm) * c.dyq +
+                                           (dkp * app + d * acc + dkm * amm))
+                         : pro;
     }
-    if (top && (which & 1)) qx[C + s] = vxo;
     if (a.ut && colok && sraw < ldx) a.temp[C + s] = tnew;   // every stored position, padding included
     dim_ = d;
     d = dip;

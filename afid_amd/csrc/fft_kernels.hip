@@ -285,6 +285,14 @@ __global__ void __launch_bounds__(256) k_yfft_forward(PoissonArgs a) {
       S[((int64_t)k * g.nzm + i) * a.ldm + m] = X;
     }
   }
+  // the padded wavenumbers m in [nymh, ldm) of each row hold zeros, so that
+  // the rows' 128-byte lines are written whole
+  const int npad = a.ldm - nymh;
+  for (int idx = threadIdx.x; idx < TK * npad; idx += 256) {
+    const int q = idx / npad, m = nymh + idx - q * npad;
+    const int k = k_of_pos(s0 + q, g.L);
+    if (k < g.nxm) S[((int64_t)k * g.nzm + i) * a.ldm + m] = mkc((R)0, (R)0);
+  }
 }
 
 // ---------------------------------------------------------------- z transforms
@@ -325,8 +333,8 @@ __global__ void __launch_bounds__(256) k_zfft(PoissonArgs a, R sign) {
   __syncthreads();
   for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
     const int q = idx % TM, nn = idx / TM;
-    const int m = m0 + q;
-    if (m < a.nymh) S[(int64_t)nn * a.ldm + m] = lds[q * N + swz(nn, zsw)];
+    const int m = m0 + q;   // m < ldm: the padded wavenumbers carry zeros, so rows are written whole
+    S[(int64_t)nn * a.ldm + m] = lds[q * N + swz(nn, zsw)];
   }
 }
 
@@ -581,6 +589,7 @@ __global__ void __launch_bounds__(256) k_xsolve_pressure(PoissonArgs a) {
   const int cnt = max(0, min(lc, nxm - k0));
   const int64_t stride = (int64_t)nzm * a.ldm;
   C *S = reinterpret_cast<C *>(a.S) + (int64_t)n * a.ldm + mc;
+  C *Sp = reinterpret_cast<C *>(a.S) + (int64_t)n * a.ldm + min(m, a.ldm - 1);   // padded wavenumbers: zeros
   double2 y[LM];
 #pragma unroll
   for (int t = 0; t < LM; t++) y[t] = widen(S[(int64_t)min(k0 + min(t, lc - 1), nxm - 1) * stride]);
@@ -672,6 +681,7 @@ __global__ void __launch_bounds__(256) k_xsolve_pressure(PoissonArgs a) {
       e.x = -w[t] * e.x;
       e.y = -w[t] * e.y;
       if (live) S[(int64_t)(k0 + t) * stride] = narrow<C>(make_double2(y[t].x + e.x, y[t].y + e.y));
+      else if (m < a.ldm) Sp[(int64_t)(k0 + t) * stride] = narrow<C>(make_double2(0.0, 0.0));
     }
   }
 }
@@ -752,8 +762,9 @@ __global__ void __launch_bounds__(256) k_yfft_inverse(PoissonArgs a) {
   __syncthreads();
   for (int idx = threadIdx.x; idx < TK * N; idx += blockDim.x) {
     const int q = idx % TK, j = idx / TK;
-    const int s = s0 + q, k = k_of_pos(s, g.L);
-    if (k >= g.nxm) continue;
+    // positions k >= nxm (their lines were zero) store 0, so that every
+    // 128-byte line of the column is written whole
+    const int s = s0 + q;
     double v;
     if (py.half) {
       C z = lds[q * H + swz(j >> 1, hsw)];
