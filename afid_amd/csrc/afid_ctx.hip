@@ -98,6 +98,12 @@ struct afid_ctx {
   // profiling
   bool prof = false;
   std::vector<PendingTiming> pending;
+  // Device steps while profiling replay a second capture of the step graph
+  // whose kernels are bracketed by event-record nodes; its event pairs are
+  // re-recorded at every replay and read after each one.
+  bool prof_capture = false;
+  hipGraphExec_t prof_graph = nullptr;
+  std::vector<PendingTiming> graph_timing;
   std::map<std::string, std::pair<double, int64_t>> prof_tot;
 };
 
@@ -486,20 +492,34 @@ struct Timed {
   const char *name;
   hipEvent_t a = nullptr, b = nullptr;
   bool on;
-  Timed(afid_ctx *cc, const char *n) : c(cc), name(n), on(cc->prof && !cc->capturing) {
+  Timed(afid_ctx *cc, const char *n) : c(cc), name(n), on(cc->prof && (!cc->capturing || cc->prof_capture)) {
     if (on) {
       hipEventCreate(&a);
       hipEventCreate(&b);
-      hipEventRecord(a, c->st);
+      record(a);
     }
+  }
+  void record(hipEvent_t e) {
+    if (c->capturing) hipEventRecordWithFlags(e, c->st, hipEventRecordExternal);   // an event-record node
+    else hipEventRecord(e, c->st);
   }
   ~Timed() {
     if (on) {
-      hipEventRecord(b, c->st);
-      c->pending.push_back({name, a, b});
+      record(b);
+      (c->capturing ? c->graph_timing : c->pending).push_back({name, a, b});
     }
   }
 };
+
+static void drop_prof_graph(afid_ctx *c) {
+  if (c->prof_graph) hipGraphExecDestroy(c->prof_graph);
+  c->prof_graph = nullptr;
+  for (auto &pt : c->graph_timing) {
+    hipEventDestroy(pt.a);
+    hipEventDestroy(pt.b);
+  }
+  c->graph_timing.clear();
+}
 
 int kerr(afid_ctx *c, hipError_t e) {
   if (e != hipSuccess) {
@@ -625,6 +645,7 @@ int afid_destroy(afid_ctx *c) {
   for (int q = 0; q < 4; q++) hipFree(c->bp[q]);
   for (int q = 0; q < 3; q++) hipFree(c->fac_dev[q]);
   if (c->step_graph) hipGraphExecDestroy(c->step_graph);
+  drop_prof_graph(c);
   hipFree(c->dstep);
   hipFree(c->mc);
   hipFree(c->fwork);
@@ -1287,7 +1308,10 @@ static int ensure_dev_step(afid_ctx *c) {
 static int enqueue_step_device(afid_ctx *c) {
   int rc;
   if (!c->cfl_fold) {
-    if ((rc = kerr(c, launch_reduce_cfl(reduce_args(c), c->st)))) return rc;
+    {
+      Timed t(c, "reduce_cfl");
+      if ((rc = kerr(c, launch_reduce_cfl(reduce_args(c), c->st)))) return rc;
+    }
     if ((rc = kerr(c, launch_step_begin(c->dstep, c->out, c->kconst, c->st)))) return rc;
   } else if ((rc = kerr(c, launch_step_begin(c->dstep, nullptr, c->kconst, c->st)))) {
     return rc;
@@ -1306,6 +1330,28 @@ static int enqueue_step_device(afid_ctx *c) {
   return kerr(c, launch_step_end(c->dstep, c->st));
 }
 
+// One step captured into a graph; its kernels take no per-step host data.
+// timed = true brackets each kernel by event-record nodes (profiling).
+static int capture_step(afid_ctx *c, hipGraphExec_t *out, bool timed) {
+  hipGraph_t graph = nullptr;
+  HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+  c->capturing = true;
+  c->prof_capture = timed;
+  int rc = enqueue_step_device(c);
+  c->capturing = false;
+  c->prof_capture = false;
+  hipError_t e = hipStreamEndCapture(c->st, &graph);
+  if (rc) {
+    if (graph) hipGraphDestroy(graph);
+    return rc;
+  }
+  HIPCHK(c, e);
+  e = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  HIPCHK(c, e);
+  return AFID_OK;
+}
+
 // Device step state from the host values, and the step graph, captured once.
 static int dev_steps_prepare(afid_ctx *c, int ntime0, int idtv, double dt, double dtmin, double dtmax,
                              double limitcfl) {
@@ -1322,27 +1368,32 @@ static int dev_steps_prepare(afid_ctx *c, int ntime0, int idtv, double dt, doubl
   }
   HIPCHK(c, hipMemcpyAsync(c->dstep, &h, sizeof(DevStep), hipMemcpyHostToDevice, c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));
-  if (!c->step_graph) {   // capture one step once; its kernels take no per-step host data
-    hipGraph_t graph = nullptr;
-    HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-    c->capturing = true;
-    rc = enqueue_step_device(c);
-    c->capturing = false;
-    hipError_t e = hipStreamEndCapture(c->st, &graph);
-    if (rc) {
-      if (graph) hipGraphDestroy(graph);
-      return rc;
-    }
-    HIPCHK(c, e);
-    e = hipGraphInstantiate(&c->step_graph, graph, nullptr, nullptr, 0);
-    hipGraphDestroy(graph);
-    HIPCHK(c, e);
+  if (!c->step_graph && (rc = capture_step(c, &c->step_graph, false))) return rc;
+  if (c->prof && !c->prof_graph && (rc = capture_step(c, &c->prof_graph, true))) {
+    drop_prof_graph(c);
+    return rc;
   }
   c->dev_steps_ready = true;
   return AFID_OK;
 }
 
+// Profiling replays the timed copy of the step graph one step at a time and
+// adds each kernel's event-pair time to the profile, as host-driven steps do.
 static int dev_steps_enqueue(afid_ctx *c, int nsteps) {
+  if (c->prof && c->prof_graph) {
+    for (int n = 0; n < nsteps; n++) {
+      HIPCHK(c, hipGraphLaunch(c->prof_graph, c->st));
+      HIPCHK(c, hipStreamSynchronize(c->st));
+      for (auto &pt : c->graph_timing) {
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, pt.a, pt.b));
+        auto &e = c->prof_tot[pt.name];
+        e.first += ms;
+        e.second += 1;
+      }
+    }
+    return AFID_OK;
+  }
   for (int n = 0; n < nsteps; n++) HIPCHK(c, hipGraphLaunch(c->step_graph, c->st));
   return AFID_OK;
 }
@@ -1473,6 +1524,7 @@ int afid_tune(afid_ctx *c, int *zm_out) {
     c->step_graph = nullptr;
     c->dev_steps_ready = false;
   }
+  drop_prof_graph(c);
   if (zm_out) *zm_out = best;
   return AFID_OK;
 }

@@ -213,11 +213,10 @@ def bench_mixed(args):
         fields[name] = {nm: c.download(nm) for nm in ("vx", "vy", "vz", "temp", "pr")}
         res[name] = {"Mcell_updates_per_s": round(cells * done / el / 1e6, 2), "ms_per_step": round(el / done * 1e3, 3),
                      "max_divergence": c.CheckDivergence(), "time": c.time}
-        # per-kernel HIP-event times over a few host-driven steps after the comparison
+        # per-kernel HIP-event times over a few graph steps after the comparison
         c.profile(True)
         c.profile_reset()
-        for n in range(args.profile_steps):
-            c.step(args.mixed_steps + n, *ctl)
+        c.steps(args.mixed_steps, args.profile_steps, *ctl)
         prof = c.profile_read()
         c.profile(False)
         res[name]["kernel_ms_per_step"] = {k: round(v[0] / args.profile_steps, 3) for k, v in prof.items() if v[1]}
@@ -329,12 +328,15 @@ def main():
     value = cells * args.steps / elapsed / 1e6
     divmax = cell.CheckDivergence()
 
-    # per-kernel HIP-event timing on the engine stream (separate short pass)
+    # per-kernel HIP-event timing on the engine stream (separate short pass):
+    # steps from a second capture of the same step graph, each kernel
+    # bracketed by event-record nodes, read after every replay
     cell.profile(True)
     cell.profile_reset()
-    for _ in range(args.profile_steps):   # host-driven steps: each kernel bracketed by HIP events
-        cell.step(ntime, *args_ctl)
-        ntime += 1
+    err, _, done = cell.steps(ntime, args.profile_steps, *args_ctl)
+    if err or done != args.profile_steps:
+        raise RuntimeError(f"profiled steps stopped early: errorcode {err} after {done} steps")
+    ntime += done
     prof = cell.profile_read()
     cell.profile(False)
     kern = {k: {"avg_ms": v[0] / v[1], "launches": v[1]} for k, v in prof.items() if v[1] > 0}

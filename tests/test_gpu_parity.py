@@ -330,6 +330,32 @@ def test_device_steps_match_host_steps(afid, c):
     dev.close()
 
 
+def test_profiled_device_steps(afid):
+    """Device steps with profiling on replay the step graph with event-record
+    nodes around each kernel: the fields equal those of unprofiled device steps
+    bit for bit, and every kernel of a step is timed once per launch."""
+    c = SMALL_CASES[0]
+    cells = [afid.Cell(afid.CellParams(nsst=3, **dict(c, alx3=c.get("alx3", 1.0)))) for _ in range(2)]
+    for x in cells:
+        x.init_linear_noise(4, 1e-3)
+        x.steps(0, 2, 1, 2e-3, 1e-4, 5e-3, 1.2)
+    plain, prof = cells
+    prof.profile(True)
+    prof.profile_reset()
+    assert plain.steps(2, 3, 1, 2e-3, 1e-4, 5e-3, 1.2) == prof.steps(2, 3, 1, 2e-3, 1e-4, 5e-3, 1.2)
+    k = prof.profile_read()
+    prof.profile(False)
+    for nm in ("vx", "vy", "vz", "temp", "pr"):
+        assert np.array_equal(plain.download(nm), prof.download(nm)), nm
+    per_step = {"reduce_cfl": 1, "stencil_rhs": 3, "column_solve": 3, "div_yfft_r2c": 3, "zfft_fwd": 3,
+                "xsolve_pressure": 3, "zfft_bwd": 3, "yfft_c2r": 3, "correct_vel_pr": 3}
+    for nm, n in per_step.items():
+        total_ms, count = k[nm]
+        assert count == 3 * n and total_ms > 0.0, (nm, k[nm])
+    for x in cells:
+        x.close()
+
+
 @pytest.mark.parametrize("fold", [False, True], ids=["reduce", "fold"])
 @pytest.mark.parametrize("c", SMALL_CASES, ids=case_id)
 def test_replayed_step_graphs_take_each_steps_cfl(afid, c, fold):
