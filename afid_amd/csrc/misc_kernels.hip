@@ -66,17 +66,60 @@ __device__ __forceinline__ void for_points(const Geom &g, int kmax, F &&f) {
   }
 }
 
-__global__ void __launch_bounds__(256) k_red_cfl(ReduceArgs a) {
+// CalcMaxCFL (CalcMaxCFL.F90:26-41) in the block shape of the stencil pass:
+// four waves on four neighbouring y columns at the same 64 stored positions
+// march through CFL_ZP z planes, with vz of plane i+1 carried in registers to
+// the next plane and vy of column j+1 taken from the neighbouring wave through
+// LDS (the group's last column loads it), so each field is read about once.
+// Each of the RBLK blocks walks a contiguous range of these tiles (XCD-
+// remapped) and leaves one maximum; the terms are summed in the reference
+// order, so the maximum equals that of the point-wise form bit for bit.
+constexpr int CFL_COLS = 4, CFL_WG = CFL_COLS * 64, CFL_ZP = 32;
+__global__ void __launch_bounds__(CFL_WG) k_red_cfl(ReduceArgs a) {
+  __shared__ double xv[2][CFL_WG];
   const Geom g = a.g;
+  const int L = g.L, ldx = g.ldx, nxm = g.nxm;
+  const int nq = (ldx + 63) / 64, ngr = (g.nym + CFL_COLS - 1) / CFL_COLS;
+  const int nib = (g.nzm + CFL_ZP - 1) / CFL_ZP;
+  const int64_t ntile = (int64_t)nq * ngr * nib;
+  const int64_t b = xcd_block(RBLK);   // the grid is RBLK blocks, a multiple of 8
+  const int64_t per = (ntile + RBLK - 1) / RBLK;
+  const int64_t t0 = b * per, t1 = min(t0 + per, ntile);
+  const int lane = (int)threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
   double m = 0.0;
-  for_points(g, g.nxm, [&](int j, int i, int64_t C, int s, int k) {
-    const int64_t CJP = col_off(wrapp(j + 1, g.nym), i, g), CIP = col_off(j, wrapp(i + 1, g.nzm), g);
-    const int skp = pos_of_k(k + 1, g.L);
-    double qcf = (fabs((a.f.vz[C + s] + a.f.vz[CIP + s]) * 0.5 * a.dz) +
-                  fabs((a.f.vy[C + s] + a.f.vy[CJP + s]) * 0.5 * a.dy) +
-                  fabs((a.f.vx[C + s] + a.f.vx[C + skp]) * 0.5 * a.kc[KC_UDX3M * g.ldx + s]));
-    m = fmax(m, qcf);
-  });
+  int par = 0;
+  for (int64_t tile = t0; tile < t1; tile++) {
+    const int q = (int)(tile % nq);
+    const int64_t rest = tile / nq;
+    const int gr = (int)(rest % ngr), ib = (int)(rest / ngr);
+    const int ncols = min(CFL_COLS, g.nym - gr * CFL_COLS);
+    const bool colok = w < ncols;
+    const int jc = colok ? gr * CFL_COLS + w : g.nym - 1;
+    const int sraw = q * 64 + lane, s = min(sraw, ldx - 1), k = k_of_pos(s, L);
+    const bool live = colok && sraw < ldx && k < nxm;
+    const int skp = pos_of_k(min(k + 1, g.nx - 1), L);
+    const bool jp_in = w + 1 < ncols;
+    const int jp = wrapp(jc + 1, g.nym);
+    const double u3m = a.kc[KC_UDX3M * ldx + s];
+    const int i0 = ib * CFL_ZP, i1 = min(i0 + CFL_ZP, g.nzm);
+    double vzc = a.f.vz[col_off(jc, i0, g) + s];
+    for (int ic = i0; ic < i1; ic++) {
+      const int64_t C = col_off(jc, ic, g);
+      const double vzip = a.f.vz[col_off(jc, wrapp(ic + 1, g.nzm), g) + s];
+      const double vyc = a.f.vy[C + s];
+      const double vxc = a.f.vx[C + s], vxk1 = a.f.vx[C + skp];
+      double vyjp = jp_in ? 0.0 : a.f.vy[col_off(jp, ic, g) + s];
+      xv[par][threadIdx.x] = vyc;
+      __syncthreads();
+      if (jp_in) vyjp = xv[par][threadIdx.x + 64];
+      par ^= 1;
+      const double qcf = (fabs((vzc + vzip) * 0.5 * a.dz) + fabs((vyc + vyjp) * 0.5 * a.dy) +
+                          fabs((vxc + vxk1) * 0.5 * u3m));
+      if (live) m = fmax(m, qcf);
+      vzc = vzip;
+    }
+  }
   m = block_reduce(m, OpMax(), 0);
   if (threadIdx.x == 0) a.partial[blockIdx.x] = m;
 }

@@ -109,19 +109,47 @@ __global__ void __launch_bounds__(256) k_factor(FactorArgs a) {
     }
     __syncthreads();
     if (threadIdx.x < SYS_N) {
+      // The LU recurrence, one thread per system.  Its operands are loaded in
+      // batches of FB rows ahead of the serial chain, which then runs in
+      // registers (one memory round trip per batch instead of per row).
+      constexpr int FB = 32;
       const int q = threadIdx.x, n = nsys(q);
-      double *dl = W(q, W_DL), *du = W(q, W_DU), *l = W(q, W_L), *u = W(q, W_U);
+      const double *dl = W(q, W_DL), *du = W(q, W_DU);
+      double *l = W(q, W_L), *u = W(q, W_U);
       l[0] = 0.0;
       u[0] = 1.0;
-      for (int k = 1; k < n; k++) {
-        // dgttrf would interchange rows here: the pivot-free LU is refused, and
-        // the cell halts before any substep of this step changes a field
-        if (fabs(u[k - 1]) < fabs(dl[k])) {
-          a.ds->pivot = 1;
-          a.ds->halted = 1;
+      double up = 1.0;   // u(k-1)
+      bool piv = false;
+      for (int k0 = 1; k0 < n; k0 += FB) {
+        double d[FB], e[FB], lo[FB], uo[FB];
+#pragma unroll
+        for (int t = 0; t < FB; t++) {
+          const int k = min(k0 + t, n - 1);
+          d[t] = dl[k];
+          e[t] = du[k - 1];
         }
-        l[k] = dl[k] / u[k - 1];
-        u[k] = 1.0 - l[k] * du[k - 1];
+#pragma unroll
+        for (int t = 0; t < FB; t++) {
+          if (k0 + t < n) {
+            // dgttrf would interchange rows here: the pivot-free LU is refused,
+            // and the cell halts before any substep of this step changes a field
+            piv = piv || fabs(up) < fabs(d[t]);
+            lo[t] = d[t] / up;
+            up = 1.0 - lo[t] * e[t];
+            uo[t] = up;
+          }
+        }
+#pragma unroll
+        for (int t = 0; t < FB; t++) {
+          if (k0 + t < n) {
+            l[k0 + t] = lo[t];
+            u[k0 + t] = uo[t];
+          }
+        }
+      }
+      if (piv) {
+        a.ds->pivot = 1;
+        a.ds->halted = 1;
       }
     }
     __syncthreads();
