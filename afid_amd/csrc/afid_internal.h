@@ -116,13 +116,22 @@ __device__ __forceinline__ unsigned long long coh_u64(const unsigned long long *
 __device__ __forceinline__ double coh_dbl(const double *p) {
   return __longlong_as_double((long long)coh_u64(reinterpret_cast<const unsigned long long *>(p)));
 }
+// The scalars are the same in every lane: each word is moved to scalar
+// registers (readfirstlane) after its coherent load, so that they do not hold
+// vector registers for the whole kernel (the stencil pass would drop from
+// three to two waves per SIMD).  Call with every lane of the wave active.
 __device__ __forceinline__ StepScalars coh_scalars(const StepScalars *p) {
   static_assert(sizeof(StepScalars) % 8 == 0, "StepScalars is read in 8-byte words");
   StepScalars s;
   const unsigned long long *src = reinterpret_cast<const unsigned long long *>(p);
   unsigned long long *dst = reinterpret_cast<unsigned long long *>(&s);
 #pragma unroll
-  for (int q = 0; q < (int)(sizeof(StepScalars) / 8); q++) dst[q] = coh_u64(src + q);
+  for (int q = 0; q < (int)(sizeof(StepScalars) / 8); q++) {
+    const unsigned long long v = coh_u64(src + q);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    dst[q] = ((unsigned long long)hi << 32) | lo;
+  }
   return s;
 }
 
