@@ -191,6 +191,129 @@ constexpr int TM = 4;   // complex lines (y wavenumbers) per z-FFT block
 // tile: (TK consecutive interleaved positions, z index i); wave w transforms
 // lines 2w and 2w+1 of the tile.  Thread t loads slot q = t % TK of the rows
 // j = t / TK + 32 it, so its k, neighbour slot and udx3m stay fixed.
+// The same stages for a line length known at compile time.  The radices
+// come in the order the host plan uses (4s, then 2s, 3s, 5s) and every
+// butterfly performs the same operations on the same twiddles, so a line
+// transformed here equals the runtime form bit for bit; with the radix, span
+// and line count constant, the index arithmetic folds and only the twiddles a
+// radix uses are loaded.
+template <int N> struct CtPlan {
+  int f[16] = {};
+  int nf = 0;
+  constexpr CtPlan() {
+    int n = N;
+    while (n % 4 == 0) { f[nf++] = 4; n /= 4; }
+    while (n % 2 == 0) { f[nf++] = 2; n /= 2; }
+    while (n % 3 == 0) { f[nf++] = 3; n /= 3; }
+    while (n % 5 == 0) { f[nf++] = 5; n /= 5; }
+  }
+};
+
+template <typename R, int N, int NL, int ST, int M>
+__device__ __forceinline__ void ct_fft_stage(cplx_t<R> *lines, const double2 *tw, R sign, int lane) {
+  using C = cplx_t<R>;
+  constexpr CtPlan<N> PL{};
+  constexpr int r = PL.f[ST];
+  constexpr int MR = M * r, nb = N / r, tws = N / MR, total = NL * nb;
+  constexpr bool sw = (N & 7) == 0;
+#pragma unroll
+  for (int w0 = 0; w0 < total; w0 += 64 * TB) {
+    C t1[TB], t2[TB], t3[TB], t4[TB];
+    int lo[TB], e0[TB];
+    bool ok[TB];
+#pragma unroll
+    for (int b = 0; b < TB; b++) {
+      if (w0 + b * 64 >= total) break;
+      const int wid = w0 + b * 64 + lane;
+      ok[b] = wid < total;
+      const int wl = ok[b] ? wid : 0;
+      const int q = (NL > 1 && wl >= nb) ? 1 : 0;
+      const int w = wl - q * nb;
+      const int bb = w / M;
+      const int u = w - bb * M;
+      lo[b] = q * N;
+      e0[b] = bb * MR + u;
+      const int t1i = u * tws;   // v * t1i < N for v < r
+      t1[b] = twid(tw, t1i, sign);
+      if constexpr (r >= 3) t2[b] = twid(tw, 2 * t1i, sign);
+      if constexpr (r >= 4) t3[b] = twid(tw, 3 * t1i, sign);
+      if constexpr (r >= 5) t4[b] = twid(tw, 4 * t1i, sign);
+    }
+#pragma unroll
+    for (int b = 0; b < TB; b++) {
+      if (w0 + b * 64 >= total) break;
+      if (!ok[b]) continue;
+      C *line = lines + lo[b];
+#define X(v) line[swz(e0[b] + (v) * M, sw)]
+      if constexpr (r == 2) {
+        C x0 = X(0), x1 = cmul(X(1), t1[b]);
+        X(0) = cadd(x0, x1);
+        X(1) = csub(x0, x1);
+      } else if constexpr (r == 4) {
+        C x0 = X(0);
+        C x1 = cmul(X(1), t1[b]);
+        C x2 = cmul(X(2), t2[b]);
+        C x3 = cmul(X(3), t3[b]);
+        C a0 = cadd(x0, x2), a1 = csub(x0, x2);
+        C b0 = cadd(x1, x3), b1 = mul_si(csub(x1, x3), sign);
+        X(0) = cadd(a0, b0);
+        X(1) = cadd(a1, b1);
+        X(2) = csub(a0, b0);
+        X(3) = csub(a1, b1);
+      } else if constexpr (r == 3) {
+        const R s3 = (R) 0.86602540378443864676372317075294;  // sin(2 pi/3)
+        C x0 = X(0);
+        C x1 = cmul(X(1), t1[b]);
+        C x2 = cmul(X(2), t2[b]);
+        C p1 = cadd(x1, x2);
+        C p2 = mul_si(csub(x1, x2), sign * s3);
+        C q0 = mkc(x0.x - (R)0.5 * p1.x, x0.y - (R)0.5 * p1.y);
+        X(0) = cadd(x0, p1);
+        X(1) = cadd(q0, p2);
+        X(2) = csub(q0, p2);
+      } else {  // r == 5
+        const R c1 = (R) 0.30901699437494742410229341718282;   // cos(2 pi/5)
+        const R c2 = (R) -0.80901699437494742410229341718282;  // cos(4 pi/5)
+        const R s1 = (R) 0.95105651629515357211643933337938;   // sin(2 pi/5)
+        const R s2 = (R) 0.58778525229247312916870595463907;   // sin(4 pi/5)
+        C x0 = X(0);
+        C x1 = cmul(X(1), t1[b]);
+        C x2 = cmul(X(2), t2[b]);
+        C x3 = cmul(X(3), t3[b]);
+        C x4 = cmul(X(4), t4[b]);
+        C p1 = cadd(x1, x4), p2 = cadd(x2, x3);
+        C p3 = csub(x1, x4), p4 = csub(x2, x3);
+        C a1 = mkc(x0.x + c1 * p1.x + c2 * p2.x, x0.y + c1 * p1.y + c2 * p2.y);
+        C a2 = mkc(x0.x + c2 * p1.x + c1 * p2.x, x0.y + c2 * p1.y + c1 * p2.y);
+        C b1 = mul_si(mkc(s1 * p3.x + s2 * p4.x, s1 * p3.y + s2 * p4.y), sign);
+        C b2 = mul_si(mkc(s2 * p3.x - s1 * p4.x, s2 * p3.y - s1 * p4.y), sign);
+        X(0) = cadd(x0, cadd(p1, p2));
+        X(1) = cadd(a1, b1);
+        X(2) = cadd(a2, b2);
+        X(3) = csub(a2, b2);
+        X(4) = csub(a1, b1);
+      }
+#undef X
+    }
+  }
+  wave_lds_sync();
+  if constexpr (ST + 1 < PL.nf) ct_fft_stage<R, N, NL, ST + 1, MR>(lines, tw, sign, lane);
+}
+
+// Dispatch on the plan's length: compile-time stages for the lengths the
+// configurations use, the runtime stages for any other.
+template <typename R, int NL>
+__device__ __forceinline__ void fft_wave(cplx_t<R> *lines, const FFTPlan &pl, R sign, int lane) {
+  switch (pl.n) {
+#define CT_CASE(NN) \
+  case NN: ct_fft_stage<R, NN, NL, 0, 1>(lines, pl.tw, sign, lane); return;
+    CT_CASE(768) CT_CASE(512) CT_CASE(384) CT_CASE(256) CT_CASE(192) CT_CASE(128) CT_CASE(96) CT_CASE(64)
+    CT_CASE(48) CT_CASE(32)
+#undef CT_CASE
+    default: lds_fft_wave<R>(lines, NL, pl, sign, lane);
+  }
+}
+
 template <typename R>
 __global__ void __launch_bounds__(256) k_yfft_forward(PoissonArgs a) {
   using C = cplx_t<R>;
@@ -253,7 +376,7 @@ __global__ void __launch_bounds__(256) k_yfft_forward(PoissonArgs a) {
     }
   }
   __syncthreads();
-  lds_fft_wave<R>(lds + (threadIdx.x / 64) * (TK / 4) * H, TK / 4, py.c, (R)-1, threadIdx.x % 64);
+  fft_wave<R, TK / 4>(lds + (threadIdx.x / 64) * (TK / 4) * H, py.c, (R)-1, threadIdx.x % 64);
   __syncthreads();
   const int nymh = a.nymh;
   for (int base = threadIdx.x; base < TK * nymh; base += 256 * LB) {
@@ -329,7 +452,7 @@ __global__ void __launch_bounds__(256) k_zfft(PoissonArgs a, R sign) {
     }
   }
   __syncthreads();
-  lds_fft_wave<R>(lds + (threadIdx.x / 64) * N, 1, a.pz, sign, threadIdx.x % 64);
+  fft_wave<R, 1>(lds + (threadIdx.x / 64) * N, a.pz, sign, threadIdx.x % 64);
   __syncthreads();
   for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
     const int q = idx % TM, nn = idx / TM;
@@ -758,7 +881,7 @@ __global__ void __launch_bounds__(256) k_yfft_inverse(PoissonArgs a) {
     }
   }
   __syncthreads();
-  lds_fft_wave<R>(lds + (threadIdx.x / 64) * (TK / 4) * H, TK / 4, py.c, (R)1, threadIdx.x % 64);
+  fft_wave<R, TK / 4>(lds + (threadIdx.x / 64) * (TK / 4) * H, py.c, (R)1, threadIdx.x % 64);
   __syncthreads();
   for (int idx = threadIdx.x; idx < TK * N; idx += blockDim.x) {
     const int q = idx % TK, j = idx / TK;
