@@ -378,43 +378,43 @@ __global__ void __launch_bounds__(256) k_yfft_forward(PoissonArgs a) {
   __syncthreads();
   fft_wave<R, TK / 4>(lds + (threadIdx.x / 64) * (TK / 4) * H, py.c, (R)-1, threadIdx.x % 64);
   __syncthreads();
-  const int nymh = a.nymh;
-  for (int base = threadIdx.x; base < TK * nymh; base += 256 * LB) {
-    C pw[LB];
-#pragma unroll
-    for (int b = 0; b < LB; b++) {
-      const int idx = min(base + 256 * b, TK * nymh - 1);
-      const int m = idx % nymh;
-      pw[b] = py.half ? cvt<R>(py.post[m]) : mkc((R)1, (R)0);
-    }
-#pragma unroll
-    for (int b = 0; b < LB; b++) {
-      const int idx = base + 256 * b;
-      if (idx >= TK * nymh) continue;
-      const int q = idx / nymh, m = idx - q * nymh;
-      const int k = k_of_pos(s0 + q, g.L);
-      if (k >= g.nxm) continue;
-      C X;
-      if (py.half) {
-        C zm = lds[q * H + swz(m == H ? 0 : m, hsw)];
-        C zc = conj2(lds[q * H + swz(m == 0 ? 0 : H - m, hsw)]);
-        C E = mkc((R)0.5 * (zm.x + zc.x), (R)0.5 * (zm.y + zc.y));
-        C D = csub(zm, zc);
-        C O = mkc((R)0.5 * D.y, (R)-0.5 * D.x);  // D / (2i)
-        X = cadd(E, cmul(pw[b], O));
-      } else {
-        X = lds[q * H + swz(m, hsw)];
-      }
-      S[((int64_t)k * g.nzm + i) * a.ldm + m] = X;
-    }
-  }
-  // the padded wavenumbers m in [nymh, ldm) of each row hold zeros, so that
-  // the rows' 128-byte lines are written whole
-  const int npad = a.ldm - nymh;
-  for (int idx = threadIdx.x; idx < TK * npad; idx += 256) {
-    const int q = idx / npad, m = nymh + idx - q * npad;
+  // Post-processing: 32 lanes per real line q, wavenumbers m = lane + 32 t.
+  // The padded wavenumbers m in [nymh, ldm) of each row store zeros, so that
+  // the rows' 128-byte lines are written whole.
+  static_assert(TK * 32 == 256, "one real line per 32 lanes");
+  {
+    const int nymh = a.nymh, ldm = a.ldm;
+    const int q = (int)threadIdx.x >> 5, lm = (int)threadIdx.x & 31;
     const int k = k_of_pos(s0 + q, g.L);
-    if (k < g.nxm) S[((int64_t)k * g.nzm + i) * a.ldm + m] = mkc((R)0, (R)0);
+    const bool rowok = k < g.nxm;
+    C *Srow = S + ((int64_t)(rowok ? k : 0) * g.nzm + i) * ldm;
+    for (int m0 = lm; m0 < ldm; m0 += 32 * LB) {
+      C pw[LB];
+#pragma unroll
+      for (int b = 0; b < LB; b++) {
+        const int m = min(m0 + 32 * b, nymh - 1);
+        pw[b] = py.half ? cvt<R>(py.post[m]) : mkc((R)1, (R)0);
+      }
+#pragma unroll
+      for (int b = 0; b < LB; b++) {
+        const int m = m0 + 32 * b;
+        if (m >= ldm) continue;
+        C X = mkc((R)0, (R)0);
+        if (m < nymh) {
+          if (py.half) {
+            C zm = lds[q * H + swz(m == H ? 0 : m, hsw)];
+            C zc = conj2(lds[q * H + swz(m == 0 ? 0 : H - m, hsw)]);
+            C E = mkc((R)0.5 * (zm.x + zc.x), (R)0.5 * (zm.y + zc.y));
+            C D = csub(zm, zc);
+            C O = mkc((R)0.5 * D.y, (R)-0.5 * D.x);  // D / (2i)
+            X = cadd(E, cmul(pw[b], O));
+          } else {
+            X = lds[q * H + swz(m, hsw)];
+          }
+        }
+        if (rowok) Srow[m] = X;
+      }
+    }
   }
 }
 
@@ -835,49 +835,52 @@ __global__ void __launch_bounds__(256) k_yfft_inverse(PoissonArgs a) {
   const int s0 = (int)(tile % ntile) * TK, i = (int)(tile / ntile);
   const bool hsw = (H & 7) == 0;
   const int nymh = a.nymh;
-  for (int base = threadIdx.x; base < TK * H; base += 256 * YILB) {
-    C xa[YILB], xb[YILB], pw[YILB];
-    int rv[YILB];
+  // Loads: 32 lanes per real line q, points t = lane + 32 u.
+  static_assert(TK * 32 == 256, "one real line per 32 lanes");
+  {
+    const int q = (int)threadIdx.x >> 5, lt = (int)threadIdx.x & 31;
+    const int kq = k_of_pos(s0 + q, g.L);
+    const bool live = kq < g.nxm;
+    const C *X = reinterpret_cast<const C *>(a.S) + ((int64_t)min(kq, g.nxm - 1) * g.nzm + i) * a.ldm;
+    for (int t0 = lt; t0 < H; t0 += 32 * YILB) {
+      C xa[YILB], xb[YILB], pw[YILB];
+      int rv[YILB];
 #pragma unroll
-    for (int b = 0; b < YILB; b++) {   // all loads of the batch first
-      const int ic = min(base + 256 * b, TK * H - 1);
-      const int q = ic / H, t = ic - q * H;
-      const int k = min(k_of_pos(s0 + q, g.L), g.nxm - 1);
-      const C *X = reinterpret_cast<const C *>(a.S) + ((int64_t)k * g.nzm + i) * a.ldm;
-      if (py.half) {
-        xa[b] = X[t];
-        xb[b] = X[H - t];
-        pw[b] = cvt<R>(py.post[t]);
-      } else {
-        xa[b] = X[min(t, nymh - 1)];
-        xb[b] = X[min(N - t, nymh - 1)];
-        pw[b] = mkc((R)1, (R)0);
-      }
-      rv[b] = py.c.rev[t];
-    }
-#pragma unroll
-    for (int b = 0; b < YILB; b++) {
-      const int idx = base + 256 * b;
-      if (idx >= TK * H) continue;
-      const int q = idx / H, t = idx - q * H;
-      const bool live = k_of_pos(s0 + q, g.L) < g.nxm;
-      C z;
-      if (py.half) {
-        C xm = xa[b], xh = xb[b];
-        if (t == 0) { xm.y = (R)0; xh.y = (R)0; }  // DC and Nyquist: imaginary parts ignored
-        C xc = conj2(xh);
-        C E = cadd(xm, xc);
-        C O = cmul(csub(xm, xc), conj2(pw[b]));
-        z = mkc(E.x - O.y, E.y + O.x);  // E + i O
-      } else {
-        if (t < nymh) {
-          z = xa[b];
-          if (t == 0) z.y = (R)0;
+      for (int b = 0; b < YILB; b++) {   // all loads of the batch first
+        const int t = min(t0 + 32 * b, H - 1);
+        if (py.half) {
+          xa[b] = X[t];
+          xb[b] = X[H - t];
+          pw[b] = cvt<R>(py.post[t]);
         } else {
-          z = conj2(xb[b]);
+          xa[b] = X[min(t, nymh - 1)];
+          xb[b] = X[min(N - t, nymh - 1)];
+          pw[b] = mkc((R)1, (R)0);
         }
+        rv[b] = py.c.rev[t];
       }
-      lds[q * H + swz(rv[b], hsw)] = live ? z : mkc((R)0, (R)0);
+#pragma unroll
+      for (int b = 0; b < YILB; b++) {
+        const int t = t0 + 32 * b;
+        if (t >= H) continue;
+        C z;
+        if (py.half) {
+          C xm = xa[b], xh = xb[b];
+          if (t == 0) { xm.y = (R)0; xh.y = (R)0; }  // DC and Nyquist: imaginary parts ignored
+          C xc = conj2(xh);
+          C E = cadd(xm, xc);
+          C O = cmul(csub(xm, xc), conj2(pw[b]));
+          z = mkc(E.x - O.y, E.y + O.x);  // E + i O
+        } else {
+          if (t < nymh) {
+            z = xa[b];
+            if (t == 0) z.y = (R)0;
+          } else {
+            z = conj2(xb[b]);
+          }
+        }
+        lds[q * H + swz(rv[b], hsw)] = live ? z : mkc((R)0, (R)0);
+      }
     }
   }
   __syncthreads();
