@@ -422,15 +422,27 @@ __global__ void __launch_bounds__(SY_WG)
     const double prm = cur.prm, prc = cur.prc;
     const double vxip = nxt.vxc, vyip = nxt.vyc, vzip = nxt.vzc, vzipm = nxt.vzm, tip = nxt.tc;
     if (stv) {
+      // The explicit terms in E: FP64, or FP32 in the mixed-precision mode (their
+      // inputs rounded from the FP64 fields); the right-hand sides, b' and the
+      // solves stay FP64.  With E = double the expressions are the FP64 ones.
+      using E = typename std::conditional<H32, float, double>::type;
+      const E Evxm = vxm, Evxc = vxc, Evxp = vxp, Evym = vym, Evyc = vyc, Evyp = vyp;
+      const E Evzm = vzm, Evzc = vzc, Evzp = vzp, Etm = tm, Etc = tc, Etp = tp;
+      const E Evxip = vxip, Evyip = vyip, Evzip = vzip, Evzipm = vzipm, Etip = tip;
+      const E Evxim = vxim, Evximp = vximp, Evyim = vyim, Evzim = vzim, Etim = tim;
+      const E Evxjp = vxjp, Evyjp = vyjp, Evyjpm = vyjpm, Evzjp = vzjp, Etjp = tjp, Evyjpim = vyjpim;
+      const E Evxjm = vxjm, Evxjmp = vxjmp, Evyjm = vyjm, Evzjm = vzjm, Etjm = tjm, Evzjmip = vzjmip;
+      const E Eudz4 = c.udz4, Eudy4 = c.udy4, Eudx3c = udx3c, Eudx3m = udx3m;
+      const E Eudzq_re = c.udzq_re, Eudyq_re = c.udyq_re, Eudzq_pe = c.udzq_pe, Eudyq_pe = c.udyq_pe;
       // ---- vx (ExplicitTermsVX.F90:55-101, ImplicitAndUpdateVX.F90:47-60)
       {
-        double hxz = (((vzip + vzipm) * (vxip + vxc)) - ((vzc + vzm) * (vxc + vxim))) * c.udz4;
-        double hxy = (((vyjp + vyjpm) * (vxjp + vxc)) - ((vyc + vym) * (vxc + vxjm))) * c.udy4;
-        double aa = vxp + vxc, bb = vxc + vxm;
-        double hxx = (aa * aa - bb * bb) * udx3c * 0.25;
-        double dzzvx = (vxim - 2.0 * vxc + vxip) * c.udzq_re;
-        double dyyvx = (vxjm - 2.0 * vxc + vxjp) * c.udyq_re;
-        double q = -(hxx + hxy + hxz) + dyyvx + dzzvx + tc;
+        E hxz = (((Evzip + Evzipm) * (Evxip + Evxc)) - ((Evzc + Evzm) * (Evxc + Evxim))) * Eudz4;
+        E hxy = (((Evyjp + Evyjpm) * (Evxjp + Evxc)) - ((Evyc + Evym) * (Evxc + Evxjm))) * Eudy4;
+        E aa = Evxp + Evxc, bb = Evxc + Evxm;
+        E hxx = (aa * aa - bb * bb) * Eudx3c * (E)0.25;
+        E dzzvx = (Evxim - (E)2.0 * Evxc + Evxip) * Eudzq_re;
+        E dyyvx = (Evxjm - (E)2.0 * Evxc + Evxjp) * Eudyq_re;
+        double q = -(hxx + hxy + hxz) + dyyvx + dzzvx + Etc;
         double dxxvx = vxp * apc + vxc * acc + vxm * amc;
         double dxp = (prc - prm) * (c.al * udx3c);
         double r = USE_RU ? c.ro * rxo : 0.0;
@@ -443,12 +455,13 @@ __global__ void __launch_bounds__(SY_WG)
       // ---- vy (ExplicitTermsVY.F90:52-98, ImplicitAndUpdateVY.F90:52-68); kmv/kpv clamps
       {
         const double vymm = k == 0 ? vyc : vym, vypp = k == nxm - 1 ? vyc : vyp;
-        double hyx = ((vxp + vxjmp) * (vypp + vyc) - (vxc + vxjm) * (vyc + vymm)) * udx3m * 0.25;
-        double yp = vyjp + vyc, ym = vyjm + vyc;
-        double hyy = (yp * yp - ym * ym) * c.udy4;
-        double hyz = ((vyip + vyc) * (vzip + vzjmip) - (vyc + vyim) * (vzc + vzjm)) * c.udz4;
-        double dyyvy = (vyjp - 2.0 * vyc + vyjm) * c.udyq_re;
-        double dzzvy = (vyip - 2.0 * vyc + vyim) * c.udzq_re;
+        const E Evymm = vymm, Evypp = vypp;
+        E hyx = ((Evxp + Evxjmp) * (Evypp + Evyc) - (Evxc + Evxjm) * (Evyc + Evymm)) * Eudx3m * (E)0.25;
+        E yp = Evyjp + Evyc, ym = Evyjm + Evyc;
+        E hyy = (yp * yp - ym * ym) * Eudy4;
+        E hyz = ((Evyip + Evyc) * (Evzip + Evzjmip) - (Evyc + Evyim) * (Evzc + Evzjm)) * Eudz4;
+        E dyyvy = (Evyjp - (E)2.0 * Evyc + Evyjm) * Eudyq_re;
+        E dzzvy = (Evyip - (E)2.0 * Evyc + Evyim) * Eudzq_re;
         double h = -(hyx + hyy + hyz) + dyyvy + dzzvy;
         double dxxvy = vypp * aps + vyc * acs + vymm * ams;
         double dyp = (prc - prjm) * c.udy_al;
@@ -461,12 +474,13 @@ __global__ void __launch_bounds__(SY_WG)
       // ---- vz (ExplicitTermsVZ.F90:56-101, ImplicitAndUpdateVZ.F90:48-64)
       {
         const double vzmm = k == 0 ? vzc : vzm, vzpp = k == nxm - 1 ? vzc : vzp;
-        double zp = vzip + vzc, zm = vzim + vzc;
-        double hzz = (zp * zp - zm * zm) * c.udz4;
-        double hzy = ((vyjp + vyjpim) * (vzjp + vzc) - (vyc + vyim) * (vzc + vzjm)) * c.udy4;
-        double hzx = ((vxp + vximp) * (vzpp + vzc) - (vxc + vxim) * (vzc + vzmm)) * udx3m * 0.25;
-        double dzzvz = (vzip - 2.0 * vzc + vzim) * c.udzq_re;
-        double dyyvz = (vzjp - 2.0 * vzc + vzjm) * c.udyq_re;
+        const E Evzmm = vzmm, Evzpp = vzpp;
+        E zp = Evzip + Evzc, zm = Evzim + Evzc;
+        E hzz = (zp * zp - zm * zm) * Eudz4;
+        E hzy = ((Evyjp + Evyjpim) * (Evzjp + Evzc) - (Evyc + Evyim) * (Evzc + Evzjm)) * Eudy4;
+        E hzx = ((Evxp + Evximp) * (Evzpp + Evzc) - (Evxc + Evxim) * (Evzc + Evzmm)) * Eudx3m * (E)0.25;
+        E dzzvz = (Evzip - (E)2.0 * Evzc + Evzim) * Eudzq_re;
+        E dyyvz = (Evzjp - (E)2.0 * Evzc + Evzjm) * Eudyq_re;
         double h = -(hzx + hzy + hzz) + dyyvz + dzzvz;
         double dxxvz = vzpp * aps + vzc * acs + vzmm * ams;
         double dzp = (prc - prim) * c.dz * c.al;
@@ -478,11 +492,11 @@ __global__ void __launch_bounds__(SY_WG)
       }
       // ---- temperature (ExplicitTermsTemp.F90:52-93, ImplicitAndUpdateTemp.F90:41-54)
       {
-        double htz = ((vzipm + vzip) * (tip + tc) - (vzm + vzc) * (tc + tim)) * c.udz4;
-        double hty = ((vyjp + vyjpm) * (tjp + tc) - (vyc + vym) * (tc + tjm)) * c.udy4;
-        double htx = ((vxp + vxc) * (tp + tc) - (vxc + vxm) * (tc + tm)) * udx3c * 0.25;
-        double dzzt = (tip - 2.0 * tc + tim) * c.udzq_pe;
-        double dyyt = (tjp - 2.0 * tc + tjm) * c.udyq_pe;
+        E htz = ((Evzipm + Evzip) * (Etip + Etc) - (Evzm + Evzc) * (Etc + Etim)) * Eudz4;
+        E hty = ((Evyjp + Evyjpm) * (Etjp + Etc) - (Evyc + Evym) * (Etc + Etjm)) * Eudy4;
+        E htx = ((Evxp + Evxc) * (Etp + Etc) - (Evxc + Evxm) * (Etc + Etm)) * Eudx3c * (E)0.25;
+        E dzzt = (Etip - (E)2.0 * Etc + Etim) * Eudzq_pe;
+        E dyyt = (Etjp - (E)2.0 * Etc + Etjm) * Eudyq_pe;
         double h = -(htx + hty + htz) + dyyt + dzzt;
         double dxxt = tp * apc + tc * acc + tm * amc;
         double r = USE_RU ? c.ro * rto : 0.0;

@@ -1,12 +1,14 @@
-"""Mixed-precision mode (AFID_PREC_MIXED): FP32 explicit-term histories and an
-FP32 FFT spectrum in the pressure solve, FP64 elsewhere, and one FP64-residual
-refinement of the projection per step.
+"""Mixed-precision mode (AFID_PREC_MIXED): the explicit terms computed in FP32
+from the FP64 fields and their histories stored in FP32, an FP32 FFT spectrum
+in the pressure solve, FP64 elsewhere (right-hand sides, the wall-normal
+factorisations and solves, the reductions), and one FP64-residual refinement
+of the projection per step.
 
 Stated error against the FP64 path on the same seeded input: after 12 steps of
 a small cell started from rest the per-field rel-L2 difference is below 2e-5
-for vx, vy, vz (measured 3.6e-6: the vx history carries the hydrostatic
+for vx, vy, vz (measured 7.8e-6: the vx history carries the hydrostatic
 buoyancy, whose FP32 rounding is large next to the still tiny velocities),
-below 1e-9 for temp (1.9e-12) and below 1e-6 for pr modulo its mean (2.6e-9),
+below 1e-9 for temp (1.1e-10) and below 1e-6 for pr modulo its mean (7.3e-9),
 and CheckDivergence stays below 1e-8.  The FP64 path is itself checked
 against the oracle elsewhere, so these bounds are the error against the oracle
 up to that path's 1e-11.  History fields round-trip through upload/download
@@ -93,8 +95,8 @@ def test_params_precision_field():
 @pytest.mark.parametrize("c", [dict(nxm=12, nym=768, nzm=6), dict(nxm=10, nym=6, nzm=750)], ids=["y768", "z750"])
 def test_mixed_long_fft_lines(c):
     """The FP32 FFT instances on a y line of 768 and a z line of 750: mixed
-    against FP64 after 8 steps of dt 5e-4 (velocities < 1e-4, temp < 1e-8 rel-L2,
-    max|div| < 1e-8)."""
+    against FP64 after 8 steps of dt 5e-4 (velocities < 1e-4, measured 1.4e-5;
+    temp < 1e-7 rel-L2, measured 1.3e-8; max|div| < 1e-8)."""
     from afid_amd.engine import Cell, CellParams, PREC_FP64, PREC_MIXED
     out = {}
     for prec in (PREC_FP64, PREC_MIXED):
@@ -109,4 +111,4 @@ def test_mixed_long_fft_lines(c):
     print("long-line mixed vs fp64", c, errs, out[PREC_MIXED]["div"], flush=True)
     for nm in ("vx", "vy", "vz"):
         assert errs[nm] < 1e-4, nm
-    assert errs["temp"] < 1e-8 and out[PREC_MIXED]["div"] < 1e-8
+    assert errs["temp"] < 1e-7 and out[PREC_MIXED]["div"] < 1e-8
