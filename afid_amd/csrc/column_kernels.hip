@@ -227,8 +227,9 @@ __device__ __forceinline__ void sth(double *base, OffT boff, double v) {
 }
 
 // Non-temporal stores (streamed past the caches' normal retention): the
-// stencil pass's b' and history stores, which no later access of that pass
-// reads (about 2 % of its time at 384x768x768).
+// stencil pass's b' and history stores and the column solve's new fields,
+// which no later access of the same pass reads (about 2-3 % of each pass's
+// time at 384x768x768; the arrays are far larger than the caches).
 template <typename OffT>
 __device__ __forceinline__ void stf_nt(double *base, OffT boff, double v) {
   __builtin_nontemporal_store(v, reinterpret_cast<double *>(reinterpret_cast<char *>(base) + boff));
@@ -603,7 +604,7 @@ __global__ void __launch_bounds__(SOLVE_WG)
       for (int t = 0; t < LM; t++)
         if (t < L) {   // slots of k >= nx (t >= cntx) are written 0
           const int k = k0 + t, s = t * P + p;
-          stf(ox, C + (OffT)s * sizeof(double), (k >= 1 && k <= nx - 2) ? b[t] + fh[s] * cin : 0.0);
+          stf_nt(ox, C + (OffT)s * sizeof(double), (k >= 1 && k <= nx - 2) ? b[t] + fh[s] * cin : 0.0);
         }
   }
   load_chunk<LM, OffT>(b, by, C, cntm, L, p);
@@ -615,7 +616,7 @@ __global__ void __launch_bounds__(SOLVE_WG)
       for (int t = 0; t < LM; t++)
         if (t < L) {
           const int k = k0 + t, s = t * P + p;
-          stf(oy, C + (OffT)s * sizeof(double), k == nx - 1 ? vytop : (k < nxm ? b[t] + fh[s] * cin : 0.0));
+          stf_nt(oy, C + (OffT)s * sizeof(double), k == nx - 1 ? vytop : (k < nxm ? b[t] + fh[s] * cin : 0.0));
         }
   }
   load_chunk<LM, OffT>(b, bz, C, cntm, L, p);
@@ -630,7 +631,7 @@ __global__ void __launch_bounds__(SOLVE_WG)
       for (int t = 0; t < LM; t++)
         if (t < L) {
           const int k = k0 + t, s = t * P + p;
-          stf(oz, C + (OffT)s * sizeof(double), k == nx - 1 ? vztop : (k < nxm ? b[t] + fh[s] * cin : 0.0));
+          stf_nt(oz, C + (OffT)s * sizeof(double), k == nx - 1 ? vztop : (k < nxm ? b[t] + fh[s] * cin : 0.0));
         }
   }
   dvz0 = __shfl(dvz0, 0, P);
@@ -650,7 +651,7 @@ __global__ void __launch_bounds__(SOLVE_WG)
           double y = b[t] + fh[s] * cin;
           y = k == 0 ? a.sc.tempbp : y;
           y = k == nx - 1 ? a.sc.temptp : y;
-          stf(tn, C + (OffT)s * sizeof(double), k < nx ? y : 0.0);
+          stf_nt(tn, C + (OffT)s * sizeof(double), k < nx ? y : 0.0);
         }
   }
 }

@@ -270,7 +270,11 @@ int afid_interp_field(afid_ctx *ctx, int field, const double *old, int nxo, int 
 
 /* ---- profiling ----------------------------------------------------------------
  * When enabled, every kernel launch is bracketed by HIP events on the
- * context's stream; totals are per kernel name. */
+ * context's stream; totals are per kernel name.  Device steps (afid_steps and
+ * the asynchronous forms) then replay a second capture of the step graph whose
+ * kernels are bracketed by event-record nodes, one step at a time, and wait
+ * for each step to read its events: the kernels are those of the timed steps,
+ * the host no longer runs ahead of the device. */
 int afid_profile_enable(afid_ctx *ctx, int on);
 /* Writes up to cap entries: names (NUL-separated into names_buf of size nbuf),
  * total milliseconds and launch counts.  Returns the number of kernels. */
