@@ -84,7 +84,6 @@ struct afid_ctx {
   bool dev_steps_ready = false;   // device step state set and the step graph captured
   bool mixed = false;             // AFID_PREC_MIXED: FP32 histories and FFT spectrum
   int zm = 0;                     // z planes per stencil block chosen by afid_tune (0: default)
-  bool fused_update = false;      // substep update in one pass (k_substep_update) rather than two
   bool cfl_fold = false;          // device steps take CalcMaxCFL from the last correction (FP64 only)
   DevPlan dpy, dpz, dpzq;
   FFTPlan pzq{};                  // Q = nzm/16-point plan of the matrix-core z transform
@@ -706,9 +705,7 @@ int afid_create(afid_ctx **out, const afid_params *p, const afid_metrics *m) {
   }
   // decided on the coefficients the solve actually uses (caller metrics included)
   c->singular_mode = zero_mode_info(c);
-  {   // AFID_UPDATE=fused selects the one-pass update where it fits
-    const char *u = std::getenv("AFID_UPDATE");
-    c->fused_update = substep_update_fits(g) && u && std::strcmp(u, "fused") == 0;
+  {
     // AFID_CFL=fold: device steps fold CalcMaxCFL into the last correction.
     // Measured at 384x768x768 that correction takes 5.8 ms against 3.8 in
     // place, more than the 1.55 ms separate reduction it saves, so by default
@@ -1122,9 +1119,6 @@ static int correct(afid_ctx *c, int which, const char *name, int ns = 0, bool de
   a.kc = c->kc;
   a.sc = scalars(c);
   a.cflbits = cfl_fold ? &c->dstep->cflbits : nullptr;
-  // after the one-pass update the new temperature waits in the fourth b buffer
-  a.ut = which == 3 && from_ustar && c->fused_update ? c->bp[3] : nullptr;
-  a.temp = c->f[AFID_TEMP];
   Timed t(c, name);
   return kerr(c, launch_correct(a, which, c->st));
 }
@@ -1163,30 +1157,23 @@ static FusedArgs update_args(afid_ctx *c, int ns, bool dev) {
 
 // cfl_fold: the correction also reduces CalcMaxCFL into the device step
 // state for the next step (the last substep of a device step).  u* then has
-// to stay apart from the corrected fields, in the b buffers; so it does for
-// the one-pass update, whose blocks read neighbouring columns' fields while
-// others update theirs.  Elsewhere the column solve writes u* into the fields
-// and the correction works in place, which moves the same words through
-// fewer distinct arrays and runs faster.
+// to stay apart from the corrected fields, in the b buffers.  Elsewhere the
+// column solve writes u* into the fields and the correction works in place,
+// which moves the same words through fewer distinct arrays and runs faster.
 static int enqueue_substep(afid_ctx *c, int ns, bool dev, bool cfl_fold = false) {
   int rc;
   FusedArgs a = update_args(c, ns, dev);
-  const bool apart = cfl_fold || c->fused_update;
+  const bool apart = cfl_fold;
   if (apart) {
     a.ux = c->bp[0]; a.uy = c->bp[1]; a.uz = c->bp[2];
   }
-  if (c->fused_update) {
-    Timed t(c, "substep_update");
-    if ((rc = kerr(c, launch_substep_update(a, c->st)))) return rc;
-  } else {
-    {
-      Timed t(c, "stencil_rhs");
-      if ((rc = kerr(c, launch_stencil_rhs(a, c->st)))) return rc;
-    }
-    {
-      Timed t(c, "column_solve");
-      if ((rc = kerr(c, launch_column_solve(a, c->st)))) return rc;
-    }
+  {
+    Timed t(c, "stencil_rhs");
+    if ((rc = kerr(c, launch_stencil_rhs(a, c->st)))) return rc;
+  }
+  {
+    Timed t(c, "column_solve");
+    if ((rc = kerr(c, launch_column_solve(a, c->st)))) return rc;
   }
   if ((rc = run_poisson(c, 1, ns, dev, apart))) return rc;
   return correct(c, 3, "correct_vel_pr", ns, dev, apart, cfl_fold);
@@ -1496,9 +1483,7 @@ int afid_tune(afid_ctx *c, int *zm_out) {
   for (int zm : cands) {
     if (zm > c->g.nzm && zm != cands[0]) break;   // a larger block than the grid adds nothing
     a.zm = zm;
-    auto launch = [&]() {
-      return c->fused_update ? launch_substep_update(a, c->st) : launch_stencil_rhs(a, c->st);
-    };
+    auto launch = [&]() { return launch_stencil_rhs(a, c->st); };
     if ((rc = kerr(c, launch()))) break;
     if ((rc = kerr(c, hipEventRecord(e0, c->st)))) break;
     for (int r = 0; r < 3 && !rc; r++) rc = kerr(c, launch());
@@ -1516,7 +1501,7 @@ int afid_tune(afid_ctx *c, int *zm_out) {
   for (int q : {AFID_RUX, AFID_RUY, AFID_RUZ, AFID_RUTEMP})
     HIPCHK(c, hipMemsetAsync(c->f[q], 0, field_bytes(c), c->st));
   for (int q = 0; q < 4; q++) HIPCHK(c, hipMemsetAsync(c->bp[q], 0, field_bytes(c), c->st));
-  HIPCHK(c, hipMemsetAsync(c->stale, 0, sizeof(double) * c->g.ncol, c->st));   // the one-pass form writes it
+  HIPCHK(c, hipMemsetAsync(c->stale, 0, sizeof(double) * c->g.ncol, c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));
   c->zm = best;
   if (c->step_graph) {   // a captured step holds the previous launch shape

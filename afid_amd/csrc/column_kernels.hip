@@ -656,419 +656,6 @@ __global__ void __launch_bounds__(SOLVE_WG)
   }
 }
 
-// ---------------------------------------------------------------- fused substep update
-// k_substep_update: the work of k_stencil_rhs and k_column_solve in one pass,
-// so that b' never leaves the chip.  A block owns one whole y column (all
-// stored positions) and marches through zplanes consecutive z planes: nws =
-// ceil(ldx/64) stencil waves hold one stored position per thread, and one
-// solver wave follows one plane behind.
-//   stencil waves, plane i: publish plane i's vx, vy, vz, T, pr in LDS,
-//     where the k-1 / k+1 values are read; compute the explicit terms,
-//     right-hand sides and history stores exactly as k_stencil_rhs, and write
-//     b' of plane i into LDS.  The loads run one plane ahead: during plane i
-//     the thread fetches plane i+2's own-column values and plane i+1's y
-//     neighbours, histories and stale row, so a whole plane of work covers
-//     their latency (z neighbours stay in registers, as in k_stencil_rhs).
-//   solver wave, during plane i: takes plane i-1's b' from LDS into
-//     registers, 16 lanes per system in the chunk layout of k_column_solve,
-//     solves vx, vy, vz together and then the temperature, whose row 1 takes
-//     the vz increment at k = 1, and writes u* and the new temperature to the
-//     b buffers.
-// The plane values and b' are double-buffered by plane parity, so one barrier
-// per plane orders everything.  The new temperature goes to the fourth b
-// buffer rather than to the field, which blocks of neighbouring columns still
-// read; the correction pass copies it over.  The vz increment at k = 1
-// becomes the column's stale row.  The coefficient rows of the stencil (eight
-// metric rows; scale, dl, du of the three systems) and the L, RINV, E rows of
-// the solves are staged in LDS.  Each affine recurrence of a solve runs per
-// chunk from a zero carry, the chunk maps are scanned over the 16 lanes, and a
-// second pass applies the carry (its running products are k_column_solve's G
-// and H rows).
-constexpr int FU_MAX_LDX = 448;   // at most 7 stencil waves + the solver wave
-enum FuCoef {
-  FC_UDX3C = 0, FC_UDX3M, FC_AMC, FC_ACC, FC_APC, FC_AMS, FC_ACS, FC_APS,
-  FC_XSC, FC_XDL, FC_XDU, FC_YSC, FC_YDL, FC_YDU, FC_TSC, FC_TDL, FC_TDU, FC_N
-};
-__host__ __device__ inline int fu_pick_lm(int L) {
-  return L <= 8 ? 8 : (L <= 16 ? 16 : (L <= 25 ? 25 : (L <= 28 ? 28 : -1)));
-}
-// Row length of the b' and factor rows in LDS: every register slot of a chunk
-// (16 * LM positions, past ldx when L < LM) plus a pad that puts the rows of
-// one plane on different banks.
-__host__ __device__ inline int fu_ldb(const Geom &g) {
-  const int lm = fu_pick_lm(g.L);
-  return (g.ldx > P * lm ? g.ldx : P * lm) + 4;
-}
-static size_t substep_update_lds(const Geom &g) {
-  const int nts = 64 * ((g.ldx + 63) / 64);
-  return sizeof(double) * ((size_t)(FC_N + 9 + 2 * 4) * fu_ldb(g) + 2 * 5 * (size_t)nts);
-}
-
-// Scans over the 16 lanes of a row through DPP row shifts (no LDS
-// permutes): the combined map of the chunks before (up) or after (down)
-// this lane, applied to a zero carry, as scan_up / scan_down compute it.
-__device__ __forceinline__ double dpp_shr(double v, int n, double fill) {
-  const long long x = __double_as_longlong(v), f = __double_as_longlong(fill);
-  int lo = (int)x, hi = (int)(x >> 32);
-  const int flo = (int)f, fhi = (int)(f >> 32);
-  switch (n) {
-    case 1: lo = __builtin_amdgcn_update_dpp(flo, lo, 0x111, 0xf, 0xf, false);
-            hi = __builtin_amdgcn_update_dpp(fhi, hi, 0x111, 0xf, 0xf, false); break;
-    case 2: lo = __builtin_amdgcn_update_dpp(flo, lo, 0x112, 0xf, 0xf, false);
-            hi = __builtin_amdgcn_update_dpp(fhi, hi, 0x112, 0xf, 0xf, false); break;
-    case 4: lo = __builtin_amdgcn_update_dpp(flo, lo, 0x114, 0xf, 0xf, false);
-            hi = __builtin_amdgcn_update_dpp(fhi, hi, 0x114, 0xf, 0xf, false); break;
-    default: lo = __builtin_amdgcn_update_dpp(flo, lo, 0x118, 0xf, 0xf, false);
-             hi = __builtin_amdgcn_update_dpp(fhi, hi, 0x118, 0xf, 0xf, false); break;
-  }
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-__device__ __forceinline__ double dpp_shl(double v, int n, double fill) {
-  const long long x = __double_as_longlong(v), f = __double_as_longlong(fill);
-  int lo = (int)x, hi = (int)(x >> 32);
-  const int flo = (int)f, fhi = (int)(f >> 32);
-  switch (n) {
-    case 1: lo = __builtin_amdgcn_update_dpp(flo, lo, 0x101, 0xf, 0xf, false);
-            hi = __builtin_amdgcn_update_dpp(fhi, hi, 0x101, 0xf, 0xf, false); break;
-    case 2: lo = __builtin_amdgcn_update_dpp(flo, lo, 0x102, 0xf, 0xf, false);
-            hi = __builtin_amdgcn_update_dpp(fhi, hi, 0x102, 0xf, 0xf, false); break;
-    case 4: lo = __builtin_amdgcn_update_dpp(flo, lo, 0x104, 0xf, 0xf, false);
-            hi = __builtin_amdgcn_update_dpp(fhi, hi, 0x104, 0xf, 0xf, false); break;
-    default: lo = __builtin_amdgcn_update_dpp(flo, lo, 0x108, 0xf, 0xf, false);
-             hi = __builtin_amdgcn_update_dpp(fhi, hi, 0x108, 0xf, 0xf, false); break;
-  }
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-__device__ __forceinline__ double row_scan_up(double A, double B) {
-#pragma unroll
-  for (int off = 1; off < P; off <<= 1) {   // lanes below off in the row take the identity (1, 0)
-    const double A2 = dpp_shr(A, off, 1.0), B2 = dpp_shr(B, off, 0.0);
-    B = A * B2 + B;
-    A = A * A2;
-  }
-  return dpp_shr(B, 1, 0.0);
-}
-__device__ __forceinline__ double row_scan_down(double A, double B) {
-#pragma unroll
-  for (int off = 1; off < P; off <<= 1) {
-    const double A2 = dpp_shl(A, off, 1.0), B2 = dpp_shl(B, off, 0.0);
-    B = A * B2 + B;
-    A = A * A2;
-  }
-  return dpp_shl(B, 1, 0.0);
-}
-
-// One system's chunk in registers: lane p of 16 holds rows k0 .. k0+LM-1.
-// The factor rows come from LDS (rL, rR, rE), each pass's rows loaded into
-// registers in one batch before its recurrence, so the chain of dependent
-// multiply-adds never waits on LDS.  Rows past the system (k >= n) hold
-// L = 0, RINV = 0, E = 0 and b = 0: they become x = 0 and are cut off from the
-// real rows (the last real row has no superdiagonal).  Register slots past
-// the chunk (t >= L) hold L = -1, RINV = 0, E = -1, b = 0, which pass both
-// recurrences' values and chunk products through unchanged.  So every lane
-// runs all LM slots with no branch and no index clamp.
-template <int LM>
-__device__ __forceinline__ void reg_chunk_solve(double (&b)[LM], const double *rL, const double *rR,
-                                                const double *rE, int p) {
-  double r1[LM], r2[LM];
-#pragma unroll
-  for (int t = 0; t < LM; t++) r1[t] = rL[t * P + p];
-  double acc = 0.0, A = 1.0;
-#pragma unroll
-  for (int t = 0; t < LM; t++) {
-    acc = b[t] - r1[t] * acc;
-    b[t] = acc;
-    A *= -r1[t];
-  }
-  const double cin = row_scan_up(A, acc);
-  double gg = 1.0;
-#pragma unroll
-  for (int t = 0; t < LM; t++) {
-    gg *= -r1[t];
-    b[t] = b[t] + gg * cin;
-  }
-#pragma unroll
-  for (int t = 0; t < LM; t++) {
-    r1[t] = rR[t * P + p];
-    r2[t] = rE[t * P + p];
-  }
-  acc = 0.0;
-  A = 1.0;
-#pragma unroll
-  for (int t = LM - 1; t >= 0; t--) {
-    acc = b[t] * r1[t] - r2[t] * acc;
-    b[t] = acc;
-    A *= -r2[t];
-  }
-  const double cin2 = row_scan_down(A, acc);
-  double hh = 1.0;
-#pragma unroll
-  for (int t = LM - 1; t >= 0; t--) {
-    hh *= -r2[t];
-    b[t] = b[t] + hh * cin2;
-  }
-}
-
-// The values a stencil thread loads for one plane ahead of its use.
-struct FuAhead {
-  double vx, vy, vz, t, pr, vzm;                         // own column, plane i+1 (k, and vz at k-1)
-};
-struct FuSide {
-  double vxjp, vyjp, vyjpm, vzjp, tjp;                   // column j+1, plane i
-  double vxjm, vxjmp, vyjm, vzjm, tjm, prjm;             // column j-1, plane i
-  double vzjmip;                                         // column j-1, plane i+1
-  double rx, ry, rz, rt, st;                             // histories and stale row, plane i
-};
-
-template <int LM, bool USE_RU, bool DEV, bool H32>
-__global__ void __launch_bounds__(512)
-    k_substep_update(FusedArgs a, const double *__restrict__ vx, const double *__restrict__ vy,
-                     const double *__restrict__ vz, const double *__restrict__ T, const double *__restrict__ pr,
-                     double *__restrict__ rux, double *__restrict__ ruy, double *__restrict__ ruz,
-                     double *__restrict__ rut, double *__restrict__ ux, double *__restrict__ uy,
-                     double *__restrict__ uz, double *__restrict__ ut, double *__restrict__ stalep,
-                     const double *__restrict__ kc, const double *__restrict__ facp) {
-  using OffT = uint32_t;
-  extern __shared__ double sh[];
-  const Geom g = a.g;
-  const int L = g.L, ldx = g.ldx, nx = g.nx, nxm = g.nxm;
-  const int nws = (ldx + 63) / 64;
-  const int nts = nws * 64;
-  const int ldb = fu_ldb(g);
-  double *cf = sh;                    // [FC_N][ldb] stencil coefficient rows
-  double *rw = cf + FC_N * ldb;       // [SYS_N][3][ldb] L, RINV, E rows of the solves
-  double *bq = rw + 9 * ldb;          // [2 parities][4][ldb] b' of vx, vy, vz, T
-  double *pub = bq + 8 * ldb;         // [2 parities][5][nts] vx, vy, vz, T, pr of the plane
-  const int zplanes = a.zm > 0 ? a.zm : ZM_DEFAULT;
-  const int nib = (g.nzm + zplanes - 1) / zplanes;
-  const int64_t tile = xcd_block((int64_t)nib * g.nym);
-  if (tile < 0) return;
-  if (DEV && coh_int(&a.ds->halted)) return;
-  const int jc = (int)(tile % g.nym), ib = (int)(tile / g.nym);
-  const int i0 = ib * zplanes, i1 = min(i0 + zplanes, g.nzm);
-  for (int idx = threadIdx.x; idx < (FC_N + 9) * ldb; idx += blockDim.x) {
-    const int r = idx / ldb, sl = idx - r * ldb;
-    double v = 0.0;
-    if (sl >= ldx) {   // register slots past the chunk (solve rows only): L = -1, RINV = 0, E = -1
-      const int f = (r - FC_N) % 3;
-      v = r < FC_N ? 0.0 : (f == 1 ? 0.0 : -1.0);
-    } else if (r < FC_XSC) {
-      const int rows[8] = {KC_UDX3C, KC_UDX3M, KC_AM3CK, KC_AC3CK, KC_AP3CK, KC_AM3SK, KC_AC3SK, KC_AP3SK};
-      v = kc[rows[r] * ldx + sl];
-    } else if (r < FC_N) {
-      const int q = r - FC_XSC, sys = q / 3, f = q - sys * 3;
-      v = facp[(sys * FR_N + (f == 0 ? FR_SCALE : (f == 1 ? FR_DL : FR_DU))) * ldx + sl];
-    } else {   // rows past the system are cut off (see reg_chunk_solve)
-      const int q = r - FC_N, sys = q / 3, f = q - sys * 3;
-      const int nsys = sys == SYS_YZ ? nxm : nx;
-      v = k_of_pos(sl, L) < nsys ? facp[(sys * FR_N + (f == 0 ? FR_L : (f == 1 ? FR_RINV : FR_E))) * ldx + sl]
-                                 : 0.0;
-    }
-    cf[r * ldb + sl] = v;
-  }
-  for (int idx = threadIdx.x; idx < 8 * ldb; idx += blockDim.x) bq[idx] = 0.0;   // padding stays 0
-  __syncthreads();
-  const OffT cb = (OffT)ldx * sizeof(double);
-  auto colb = [&](int jj, int ii) { return ((OffT)ii * (OffT)g.nym + (OffT)jj) * cb; };
-  const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
-  if (w == nws) {
-    // ---- solver wave, one plane behind the stencil waves
-    const int lane = (int)threadIdx.x & 63;
-    const int grp = lane >> 4, p = lane & (P - 1);
-    const int sys = grp == 0 ? SYS_X : (grp == 3 ? SYS_T : SYS_YZ);
-    const int k0 = p * L;
-    const double *rL = rw + (sys * 3 + 0) * ldb, *rR = rw + (sys * 3 + 1) * ldb, *rE = rw + (sys * 3 + 2) * ldb;
-    double *outp = grp == 0 ? ux : (grp == 1 ? uy : (grp == 2 ? uz : ut));
-    for (int ic = i0; ic <= i1; ic++) {
-      __syncthreads();   // plane ic published (ic < i1); b' of plane ic-1 complete
-      if (ic == i0) continue;
-      const int icp = ic - 1;
-      const double *bsrc = bq + ((icp & 1) * 4 + grp) * ldb;
-      double b[LM];
-#pragma unroll
-      for (int t = 0; t < LM; t++) b[t] = bsrc[t * P + p];   // 0 past the system and the chunk
-      const OffT C = colb(jc, icp);
-      double dvz0 = 0.0;
-      // phase 0: vx, vy, vz; phase 1: temperature, whose row 1 takes the vz increment
-      for (int ph = 0; ph < 2; ph++) {
-        if (ph == 1) {
-          dvz0 = __shfl(dvz0, 32, 64);   // lane 32: the vz system's chunk 0
-          if (grp == 3 && p == 0) b[0] += dvz0;
-        }
-        if (ph == 0 ? grp == 3 : grp != 3) continue;
-        reg_chunk_solve<LM>(b, rL, rR, rE, p);
-#pragma unroll
-        for (int t = 0; t < LM; t++)
-          if (t < L) {
-            const int k = k0 + t, sl = t * P + p;
-            double v;
-            if (grp == 0) {
-              v = (k >= 1 && k <= nx - 2) ? b[t] : 0.0;
-            } else if (grp == 3) {
-              v = k == 0 ? a.sc.tempbp : (k == nx - 1 ? a.sc.temptp : b[t]);
-              v = k < nx ? v : 0.0;
-            } else {
-              // row nx is outside the nxm system and keeps the field's value,
-              // which u* carries to the correction
-              v = k == nx - 1 ? ldf(grp == 1 ? vy : vz, C + (OffT)sl * sizeof(double)) : (k < nxm ? b[t] : 0.0);
-            }
-            stf(outp, C + (OffT)sl * sizeof(double), v);
-          }
-        if (grp == 2 && p == 0) dvz0 = b[0] - coh_dbl(vz + (size_t)C / sizeof(double));   // k = 1: position 0
-      }
-      if (lane == 0) stalep[(int64_t)icp * g.nym + jc] = dvz0;
-    }
-    return;
-  }
-  // ---- stencil waves: thread = stored position
-  const StepScalars c = DEV ? coh_scalars(&a.ds->sc[a.ns - 1]) : a.sc;
-  const bool keep_ru = a.keep_ru != 0;
-  const int tid = (int)threadIdx.x;
-  const int s = min(tid, ldx - 1);
-  const int k = k_of_pos(s, L);
-  const bool live = tid < ldx && k < nx;
-  const int sm = k >= 1 ? pos_of_k(k - 1, L) : s;
-  const int sp = k + 1 < nx ? pos_of_k(k + 1, L) : s;
-  const OffT sb = (OffT)s * sizeof(double), smb = (OffT)sm * sizeof(double), spb = (OffT)sp * sizeof(double);
-  const bool inx = k >= 1 && k <= nx - 2;
-  const bool iny = k <= nxm - 1;
-  const int jm = wrapm(jc - 1, g.nym), jp = wrapp(jc + 1, g.nym);
-  auto cfv = [&](int r) { return cf[r * ldb + s]; };
-  auto load_ahead = [&](int ii) {
-    const OffT CI = colb(jc, ii);
-    FuAhead n;
-    n.vx = ldf(vx, CI + sb); n.vy = ldf(vy, CI + sb); n.vz = ldf(vz, CI + sb); n.t = ldf(T, CI + sb);
-    n.pr = ldf(pr, CI + sb); n.vzm = ldf(vz, CI + smb);
-    return n;
-  };
-  auto load_side = [&](int ii) {   // plane ii's y neighbours, histories and stale row
-    const int iip = wrapp(ii + 1, g.nzm);
-    const OffT CJP = colb(jp, ii), CJM = colb(jm, ii), C = colb(jc, ii);
-    FuSide y;
-    y.vxjp = ldf(vx, CJP + sb); y.vyjp = ldf(vy, CJP + sb); y.vyjpm = ldf(vy, CJP + smb);
-    y.vzjp = ldf(vz, CJP + sb); y.tjp = ldf(T, CJP + sb);
-    y.vxjm = ldf(vx, CJM + sb); y.vxjmp = ldf(vx, CJM + spb); y.vyjm = ldf(vy, CJM + sb);
-    y.vzjm = ldf(vz, CJM + sb); y.tjm = ldf(T, CJM + sb); y.prjm = ldf(pr, CJM + sb);
-    y.vzjmip = ldf(vz, colb(jm, iip) + sb);
-    y.rx = USE_RU ? ldh<H32>(rux, C + sb) : 0.0;
-    y.ry = USE_RU ? ldh<H32>(ruy, C + sb) : 0.0;
-    y.rz = USE_RU ? ldh<H32>(ruz, C + sb) : 0.0;
-    y.rt = USE_RU ? ldh<H32>(rut, C + sb) : 0.0;
-    y.st = (k == 0 && tid < ldx) ? coh_dbl(stalep + (int64_t)ii * g.nym + jc) : 0.0;
-    return y;
-  };
-  // plane i0 (centre), i0-1 (z neighbours carried in registers) and i0+1
-  const OffT C0 = colb(jc, i0);
-  double vxc = ldf(vx, C0 + sb), vyc = ldf(vy, C0 + sb), vzc = ldf(vz, C0 + sb), tc = ldf(T, C0 + sb),
-         prc = ldf(pr, C0 + sb);
-  const int ib0 = wrapm(i0 - 1, g.nzm);
-  const OffT CM = colb(jc, ib0);
-  double vxim = ldf(vx, CM + sb), vximp = ldf(vx, CM + spb), vyim = ldf(vy, CM + sb);
-  double vzim = ldf(vz, CM + sb), tim = ldf(T, CM + sb), prim = ldf(pr, CM + sb);
-  double vyjpim = ldf(vy, colb(jp, ib0) + sb);
-  FuAhead nx1 = load_ahead(wrapp(i0 + 1, g.nzm));
-  FuSide y0 = load_side(i0);
-  for (int ic = i0; ic < i1; ic++) {
-    const int ip2 = wrapp(wrapp(ic + 1, g.nzm) + 1, g.nzm);
-    const FuAhead nx2 = load_ahead(ip2);                 // plane ic+2
-    const FuSide y1 = load_side(wrapp(ic + 1, g.nzm));  // plane ic+1
-    const OffT C = colb(jc, ic);
-    double *own = pub + (ic & 1) * 5 * nts;
-    own[0 * nts + tid] = vxc;
-    own[1 * nts + tid] = vyc;
-    own[2 * nts + tid] = vzc;
-    own[3 * nts + tid] = tc;
-    own[4 * nts + tid] = prc;
-    __syncthreads();   // plane ic published; the solver takes plane ic-1's b'
-    const double vxm = own[0 * nts + sm], vxp = own[0 * nts + sp];
-    const double vym = own[1 * nts + sm], vyp = own[1 * nts + sp];
-    const double vzm = own[2 * nts + sm], vzp = own[2 * nts + sp];
-    const double tm = own[3 * nts + sm], tp = own[3 * nts + sp];
-    const double prm = own[4 * nts + sm];
-    const double vxip = nx1.vx, vyip = nx1.vy, vzip = nx1.vz, vzipm = nx1.vzm, tip = nx1.t;
-    const double vxjp = y0.vxjp, vyjp = y0.vyjp, vyjpm = y0.vyjpm, vzjp = y0.vzjp, tjp = y0.tjp;
-    const double vxjm = y0.vxjm, vxjmp = y0.vxjmp, vyjm = y0.vyjm, vzjm = y0.vzjm, tjm = y0.tjm;
-    const double prjm = y0.prjm, vzjmip = y0.vzjmip;
-    double *bqc = bq + (ic & 1) * 4 * ldb;
-    if (live) {
-      const double udx3c = cfv(FC_UDX3C), udx3m = cfv(FC_UDX3M);
-      // ---- vx (ExplicitTermsVX.F90:55-101, ImplicitAndUpdateVX.F90:47-60)
-      {
-        double hxz = (((vzip + vzipm) * (vxip + vxc)) - ((vzc + vzm) * (vxc + vxim))) * c.udz4;
-        double hxy = (((vyjp + vyjpm) * (vxjp + vxc)) - ((vyc + vym) * (vxc + vxjm))) * c.udy4;
-        double aa = vxp + vxc, bb = vxc + vxm;
-        double hxx = (aa * aa - bb * bb) * udx3c * 0.25;
-        double dzzvx = (vxim - 2.0 * vxc + vxip) * c.udzq_re;
-        double dyyvx = (vxjm - 2.0 * vxc + vxjp) * c.udyq_re;
-        double q = -(hxx + hxy + hxz) + dyyvx + dzzvx + tc;
-        double dxxvx = vxp * cfv(FC_APC) + vxc * cfv(FC_ACC) + vxm * cfv(FC_AMC);
-        double dxp = (prc - prm) * (c.al * udx3c);
-        double r = USE_RU ? c.ro * y0.rx : 0.0;
-        double rhs = (c.ga * q + r + c.alre * dxxvx - dxp) * c.dt;
-        double vin = rhs * cfv(FC_XSC) + ((cfv(FC_XDL) * vxm + vxc) + cfv(FC_XDU) * vxp);
-        if (keep_ru) sth<H32>(rux, C + sb, inx ? q : 0.0);
-        bqc[0 * ldb + s] = inx ? vin : y0.st + vxc;
-      }
-      // ---- vy (ExplicitTermsVY.F90:52-98, ImplicitAndUpdateVY.F90:52-68); kmv/kpv clamps
-      {
-        const double vymm = k == 0 ? vyc : vym, vypp = k == nxm - 1 ? vyc : vyp;
-        double hyx = ((vxp + vxjmp) * (vypp + vyc) - (vxc + vxjm) * (vyc + vymm)) * udx3m * 0.25;
-        double yp = vyjp + vyc, ym = vyjm + vyc;
-        double hyy = (yp * yp - ym * ym) * c.udy4;
-        double hyz = ((vyip + vyc) * (vzip + vzjmip) - (vyc + vyim) * (vzc + vzjm)) * c.udz4;
-        double dyyvy = (vyjp - 2.0 * vyc + vyjm) * c.udyq_re;
-        double dzzvy = (vyip - 2.0 * vyc + vyim) * c.udzq_re;
-        double h = -(hyx + hyy + hyz) + dyyvy + dzzvy;
-        double dxxvy = vypp * cfv(FC_APS) + vyc * cfv(FC_ACS) + vymm * cfv(FC_AMS);
-        double dyp = (prc - prjm) * c.udy_al;
-        double r = USE_RU ? c.ro * y0.ry : 0.0;
-        double rhs = (c.ga * h + r + c.alre * dxxvy - dyp) * c.dt;
-        if (keep_ru) sth<H32>(ruy, C + sb, iny ? h : 0.0);
-        // row nx is outside the nxm system: b' = 0 there, as the solve expects
-        bqc[1 * ldb + s] = iny ? rhs * cfv(FC_YSC) + ((cfv(FC_YDL) * vym + vyc) + cfv(FC_YDU) * vyp) : 0.0;
-      }
-      // ---- vz (ExplicitTermsVZ.F90:56-101, ImplicitAndUpdateVZ.F90:48-64)
-      {
-        const double vzmm = k == 0 ? vzc : vzm, vzpp = k == nxm - 1 ? vzc : vzp;
-        double zp = vzip + vzc, zm = vzim + vzc;
-        double hzz = (zp * zp - zm * zm) * c.udz4;
-        double hzy = ((vyjp + vyjpim) * (vzjp + vzc) - (vyc + vyim) * (vzc + vzjm)) * c.udy4;
-        double hzx = ((vxp + vximp) * (vzpp + vzc) - (vxc + vxim) * (vzc + vzmm)) * udx3m * 0.25;
-        double dzzvz = (vzip - 2.0 * vzc + vzim) * c.udzq_re;
-        double dyyvz = (vzjp - 2.0 * vzc + vzjm) * c.udyq_re;
-        double h = -(hzx + hzy + hzz) + dyyvz + dzzvz;
-        double dxxvz = vzpp * cfv(FC_APS) + vzc * cfv(FC_ACS) + vzmm * cfv(FC_AMS);
-        double dzp = (prc - prim) * c.dz * c.al;
-        double r = USE_RU ? c.ro * y0.rz : 0.0;
-        double rhs = (c.ga * h + r + c.alre * dxxvz - dzp) * c.dt;
-        if (keep_ru) sth<H32>(ruz, C + sb, iny ? h : 0.0);
-        bqc[2 * ldb + s] = iny ? rhs * cfv(FC_YSC) + ((cfv(FC_YDL) * vzm + vzc) + cfv(FC_YDU) * vzp) : 0.0;
-      }
-      // ---- temperature (ExplicitTermsTemp.F90:52-93, ImplicitAndUpdateTemp.F90:41-54)
-      {
-        double htz = ((vzipm + vzip) * (tip + tc) - (vzm + vzc) * (tc + tim)) * c.udz4;
-        double hty = ((vyjp + vyjpm) * (tjp + tc) - (vyc + vym) * (tc + tjm)) * c.udy4;
-        double htx = ((vxp + vxc) * (tp + tc) - (vxc + vxm) * (tc + tm)) * udx3c * 0.25;
-        double dzzt = (tip - 2.0 * tc + tim) * c.udzq_pe;
-        double dyyt = (tjp - 2.0 * tc + tjm) * c.udyq_pe;
-        double h = -(htx + hty + htz) + dyyt + dzzt;
-        double dxxt = tp * cfv(FC_APC) + tc * cfv(FC_ACC) + tm * cfv(FC_AMC);
-        double r = USE_RU ? c.ro * y0.rt : 0.0;
-        double rhs = (c.ga * h + r + c.alpec * dxxt) * c.dt;
-        double vin = rhs * cfv(FC_TSC) + ((cfv(FC_TDL) * tm + tc) + cfv(FC_TDU) * tp);
-        if (keep_ru) sth<H32>(rut, C + sb, inx ? h : 0.0);
-        bqc[3 * ldb + s] = inx ? vin : tc;   // row 1 gets the vz increment added by the solver
-      }
-    }
-    vxim = vxc; vximp = vxp; vyim = vyc; vzim = vzc; tim = tc; prim = prc;
-    vyjpim = vyjp;
-    vxc = nx1.vx; vyc = nx1.vy; vzc = nx1.vz; tc = nx1.t; prc = nx1.pr;
-    nx1 = nx2;
-    y0 = y1;
-  }
-  __syncthreads();     // b' of the last plane complete
-}
-
 // ---------------------------------------------------------------- per-routine kernels
 // One thread per stored point; k recovered from the interleaved position.
 struct PointIdx {
@@ -1308,7 +895,6 @@ __global__ void __launch_bounds__(SY_WG) k_correct(CorrectArgs a, int which, int
       if (!jp_in) uyjp = uy[col_off(jp, ic, g) + s];
     }
     if (which & 2) pro = qp[C + s];
-    const double tnew = a.ut && colok && sraw < ldx ? a.ut[C + s] : 0.0;
     double *buf = xv + par * (2 * SY_WG);
     buf[tid] = d;
     if (CFL) buf[SY_WG + tid] = vyo;
@@ -1339,7 +925,6 @@ __global__ void __launch_bounds__(SY_WG) k_correct(CorrectArgs a, int which, int
                                            (dkp * app + d * acc + dkm * amm))
                          : pro;
     }
-    if (a.ut && colok && sraw < ldx) a.temp[C + s] = tnew;   // every stored position, padding included
     dim_ = d;
     d = dip;
     uzc = uzip;
@@ -1389,16 +974,6 @@ hipError_t column_configure_lds(size_t bytes) {
   SETC(uint32_t, true) SETC(uint32_t, false) SETC(uint64_t, true) SETC(uint64_t, false)
 #undef SETC
 #undef SETC1
-#define SETU1(M, R, D, H)                                                                                \
-  e = hipFuncSetAttribute((const void *)k_substep_update<M, R, D, H>,                                   \
-                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);                       \
-  if (e != hipSuccess) return e;
-#define SETU(M, R) SETU1(M, R, true, false) SETU1(M, R, false, false) SETU1(M, R, true, true) SETU1(M, R, false, true)
-#define SETUM(M) SETU(M, true) SETU(M, false)
-  SETUM(8) SETUM(16) SETUM(25) SETUM(28)
-#undef SETUM
-#undef SETU
-#undef SETU1
   return hipSuccess;
 }
 
@@ -1427,37 +1002,6 @@ hipError_t launch_stencil_rhs(const FusedArgs &a, hipStream_t st) {
   SR(uint32_t, true) SR(uint32_t, false) SR(uint64_t, true) SR(uint64_t, false)
 #undef SR
 #undef SR1
-  return hipErrorInvalidValue;
-}
-
-bool substep_update_fits(const Geom &g) {
-  return g.ldx <= FU_MAX_LDX && fu_pick_lm(g.L) > 0 && substep_update_lds(g) <= 160 * 1024 &&
-         (uint64_t)g.ncol * g.ldx * sizeof(double) < (1ull << 32);
-}
-
-hipError_t launch_substep_update(const FusedArgs &a, hipStream_t st) {
-  const Geom &g = a.g;
-  if (!substep_update_fits(g)) return hipErrorInvalidValue;
-  const int zplanes = a.zm > 0 ? a.zm : ZM_DEFAULT;
-  const int64_t ntile = (int64_t)((g.nzm + zplanes - 1) / zplanes) * g.nym;
-  const dim3 grid((unsigned)(((ntile + 7) / 8) * 8));
-  const dim3 block((unsigned)(64 * ((g.ldx + 63) / 64 + 1)));
-  const size_t shm = substep_update_lds(g);
-  const bool ru = a.sc.use_ru != 0, dev = a.ds != nullptr, h32 = a.h32 != 0;
-  const int lm = fu_pick_lm(g.L);
-#define SU1(M, R, D, H)                                                                                  \
-  if (M == lm && R == ru && D == dev && H == h32) {                                                     \
-    hipLaunchKernelGGL((k_substep_update<M, R, D, H>), grid, block, shm, st, a, a.f.vx, a.f.vy, a.f.vz,   \
-                       a.f.temp, a.f.pr, a.rux, a.ruy, a.ruz, a.rut, a.bx, a.by, a.bz, a.bt, a.stale, a.kc,  \
-                       a.fac);                                                                         \
-    return hipGetLastError();                                                                          \
-  }
-#define SU(M, R) SU1(M, R, true, false) SU1(M, R, false, false) SU1(M, R, true, true) SU1(M, R, false, true)
-#define SUM(M) SU(M, true) SU(M, false)
-  SUM(8) SUM(16) SUM(25) SUM(28)
-#undef SUM
-#undef SU
-#undef SU1
   return hipErrorInvalidValue;
 }
 
@@ -1525,7 +1069,6 @@ hipError_t launch_correct(const CorrectArgs &a, int which, hipStream_t st) {
 int max_supported_L() { return 80; }
 size_t column_lds_need(const Geom &g) {
   size_t b = std::max(stencil_lds(g), solve_lds(g));
-  if (substep_update_fits(g)) b = std::max(b, substep_update_lds(g));
   return b;
 }
 

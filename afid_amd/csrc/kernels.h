@@ -53,8 +53,6 @@ struct CorrectArgs {
   const DevStep *ds;
   int ns;
   unsigned long long *cflbits;  // non-null: CalcMaxCFL of the corrected velocities, atomic max of the bits
-  const double *ut;             // non-null: the new temperature, copied into temp (every stored position)
-  double *temp;
 };
 
 // One complex FFT length with its radix stages, twiddles exp(-2 pi i t/n)
@@ -101,10 +99,6 @@ struct PoissonArgs {
 
 hipError_t launch_stencil_rhs(const FusedArgs &a, hipStream_t st);
 hipError_t launch_column_solve(const FusedArgs &a, hipStream_t st);
-// k_stencil_rhs + k_column_solve in one pass (u* and the new temperature into
-// the b buffers); for column pitches up to 896 (substep_update_fits).
-hipError_t launch_substep_update(const FusedArgs &a, hipStream_t st);
-bool substep_update_fits(const Geom &g);
 hipError_t launch_solve_update(const SolveArgs &a, int which, hipStream_t st);
 hipError_t launch_explicit(const RoutineArgs &a, int which, hipStream_t st);
 hipError_t launch_rhs(const RoutineArgs &a, int which, hipStream_t st);

@@ -27,9 +27,8 @@ HBM_PEAK_GBS = 8000.0
 # Algorithmic FP64 words per cell of each kernel: the distinct fields it must
 # read and write once (8 B each).  stencil_rhs reads vx,vy,vz,T,pr and the four
 # histories and writes the histories and the four right-hand sides b + A q_old;
-# column_solve reads those and writes the four new fields (u* in place);
-# substep_update (AFID_UPDATE=fused) is the two in one pass, writing u* and the
-# new temperature apart from the fields.  Spectral arrays hold nym/2+1 complex
+# column_solve reads those and writes the four new fields (u* in place).
+# Spectral arrays hold nym/2+1 complex
 # values per y line, one word per cell.  The correction reads u*, dph and pr and
 # writes the velocities and pr in place.  The substep figure is the reference's
 # per-routine compulsory traffic: explicit VX 5, VY 4, VZ 4, Temp 5; implicit
@@ -42,7 +41,6 @@ WORDS = {
     # HIP-event average covers all three.
     "stencil_rhs": (13 + 17 + 13) / 3,
     "column_solve": 4 + 4,
-    "substep_update": (13 + 17 + 13) / 3,
     "div_yfft_r2c": 3 + 1,
     "zfft_fwd": 2,
     "xsolve_pressure": 2,
