@@ -249,15 +249,17 @@ __device__ __forceinline__ void sth_nt(double *base, OffT boff, double v) {
 // (ExplicitTerms*, ImplicitAndUpdate* RHS loops), the history update, and
 // b' = ackl_b*rhs + A q_old, where A is the unit-diagonal normalised system of
 // SolveImpEqnUpdate_*; identity rows store q_old (+ rhs(1) = the stale row for
-// vx).  A block has SY_COLS waves on SY_COLS neighbouring y columns, all at
-// the same 64 stored positions, and marches through zplanes consecutive z
-// planes: each thread keeps its position's values of planes i-1, i and i+1 in
-// registers, so the z neighbours cost no loads.  Each wave publishes the plane
-// values its y neighbours need in LDS (double-buffered by plane, one barrier
-// per plane), so only the group's two edge columns read a y neighbour from
-// memory.  Blocks are ordered position-range fastest, then y group, then z
-// range (XCD-remapped), so an edge column is being read by the neighbouring
-// group's block on the same XCD at the same time.  A plane's loads issue in
+// vx).  A block holds ST_GC = 16 neighbouring y columns, all at the same
+// ST_SP = 16 stored positions (one 128-byte line of each column; a wave covers
+// four columns), and marches through zplanes consecutive z planes: each thread
+// keeps its position's values of planes i-1, i and i+1 in registers, so the z
+// neighbours cost no loads.  Each thread publishes the plane values its y
+// neighbours need in LDS (double-buffered by plane, one barrier per plane), so
+// only the group's two edge columns read a y neighbour from memory.  A column
+// holds ldx = 16 L stored positions, so its L tiles are all full.  Blocks are
+// ordered position-range fastest, then y group, then z range (XCD-remapped),
+// so the k+-1 lines a tile reads belong to tiles running on the same XCD at
+// the same time.  A plane's loads issue in
 // one batch before any value is used; the history rows outside the reference
 // loops (k = 1 and nx for vx/temp, nx for vy/vz), which nothing reads, are
 // kept at 0.  Histories that no later substep reads (keep_ru = 0: RK3
@@ -272,8 +274,9 @@ __device__ __forceinline__ void sth_nt(double *base, OffT boff, double v) {
 // increment at k=1, q_new - q_old, becomes the new stale row and the
 // temperature's rhs(1).
 constexpr int ZM_DEFAULT = 32;   // z planes per block unless afid_tune chose another count
-constexpr int SY_COLS = 4;       // y columns per block, one wave each
-constexpr int SY_WG = SY_COLS * 64;
+constexpr int SY_WG = 256;             // threads per block of the stencil and correction passes
+constexpr int ST_SP = 16;              // stored positions per column tile (one 128-byte line)
+constexpr int ST_GC = SY_WG / ST_SP;   // y columns per block, four per wave
 // Coefficient rows staged in LDS by k_stencil_rhs: eight per-k metric rows,
 // then scale, dl, du of the x, y/z and temperature systems.
 enum StencilCoef {
@@ -306,7 +309,7 @@ __device__ __forceinline__ Plane load_plane(const double *__restrict__ vx, const
 // Plane values a wave publishes for its y neighbours (the k-1 value of vy and
 // the k+1 value of vx are those at the neighbour's own position s).
 enum { SV_VX = 0, SV_VY, SV_VYM, SV_VZ, SV_T, SV_VXP, SV_PR, SV_VZIP, SV_N };
-static size_t stencil_lds(const Geom &) { return sizeof(double) * (SC_N * 64 + 2 * SV_N * SY_WG); }
+static size_t stencil_lds(const Geom &) { return sizeof(double) * (SC_N * ST_SP + 2 * SV_N * SY_WG); }
 
 // DEV: the substep scalars come from the device step state (a.ds) rather
 // than from the argument block; two instantiations keep each one's scalar
@@ -320,12 +323,12 @@ __global__ void __launch_bounds__(SY_WG)
                   double *__restrict__ bz, double *__restrict__ bt, const double *__restrict__ stalep,
                   const double *__restrict__ kc, const double *__restrict__ facp) {
   extern __shared__ double sh[];
-  double *cf = sh;                    // [SC_N][64] coefficient rows of the block's positions
-  double *xv = sh + SC_N * 64;        // [2][SV_N][SY_WG] published plane values
+  double *cf = sh;                    // [SC_N][ST_SP] coefficient rows of the block's positions
+  double *xv = sh + SC_N * ST_SP;     // [2][SV_N][SY_WG] published plane values
   const Geom g = a.g;
   const int L = g.L, ldx = g.ldx, nx = g.nx, nxm = g.nxm;
-  const int nq = (ldx + 63) / 64;
-  const int ngr = (g.nym + SY_COLS - 1) / SY_COLS;
+  const int nq = (ldx + ST_SP - 1) / ST_SP;
+  const int ngr = (g.nym + ST_GC - 1) / ST_GC;
   const int zplanes = a.zm > 0 ? a.zm : ZM_DEFAULT;
   const int nib = (g.nzm + zplanes - 1) / zplanes;
   const int64_t tile = xcd_block((int64_t)nib * ngr * nq);
@@ -336,18 +339,18 @@ __global__ void __launch_bounds__(SY_WG)
   if (DEV && coh_int(&a.ds->halted)) return;
   const StepScalars c = DEV ? coh_scalars(&a.ds->sc[a.ns - 1]) : a.sc;
   const bool keep_ru = a.keep_ru != 0;
-  for (int idx = threadIdx.x; idx < SC_N * 64; idx += SY_WG) {
-    const int r = idx / 64, sl = min(q * 64 + idx - r * 64, ldx - 1);
+  for (int idx = threadIdx.x; idx < SC_N * ST_SP; idx += SY_WG) {
+    const int r = idx / ST_SP, sl = min(q * ST_SP + idx - r * ST_SP, ldx - 1);
     const int src = r < SC_XSC ? kc_row_of(r) * ldx : (((r - SC_XSC) / 3) * FR_N + fac_row_of((r - SC_XSC) % 3)) * ldx;
     cf[idx] = (r < SC_XSC ? kc : facp)[src + sl];
   }
   __syncthreads();
-  const int lane = (int)threadIdx.x & 63;
-  const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
-  const int ncols = min(SY_COLS, g.nym - gr * SY_COLS);   // columns of this group
+  const int lane = (int)threadIdx.x & (ST_SP - 1);        // position within the tile
+  const int w = (int)threadIdx.x / ST_SP;                  // column within the group
+  const int ncols = min(ST_GC, g.nym - gr * ST_GC);        // columns of this group
   const bool colok = w < ncols;
-  const int jc = colok ? gr * SY_COLS + w : g.nym - 1;    // idle waves read a valid column, store nothing
-  const int sraw = q * 64 + lane;
+  const int jc = colok ? gr * ST_GC + w : g.nym - 1;       // idle threads read a valid column, store nothing
+  const int sraw = q * ST_SP + lane;
   const int s = min(sraw, ldx - 1);
   const int k = k_of_pos(s, L);
   // Every stored position of a column is written, the padding (k >= nx) as
@@ -364,7 +367,7 @@ __global__ void __launch_bounds__(SY_WG)
   const OffT spb = (OffT)(k + 1 < nx ? pos_of_k(k + 1, L) : s) * sizeof(double);
   const bool inx = k >= 1 && k <= nx - 2;
   const bool iny = k <= nxm - 1;
-  auto cfv = [&](int r) { return cf[r * 64 + lane]; };
+  auto cfv = [&](int r) { return cf[r * ST_SP + lane]; };
   const double udx3c = cfv(SC_UDX3C), udx3m = cfv(SC_UDX3M);
   const double amc = cfv(SC_AMC), acc = cfv(SC_ACC), apc = cfv(SC_APC);
   const double ams = cfv(SC_AMS), acs = cfv(SC_ACS), aps = cfv(SC_APS);
@@ -407,12 +410,12 @@ __global__ void __launch_bounds__(SY_WG)
     buf[SV_VZIP * SY_WG + tid] = nxt.vzc;
     __syncthreads();
     if (jp_in) {
-      const int o = tid + 64;
+      const int o = tid + ST_SP;
       vxjp = buf[SV_VX * SY_WG + o]; vyjp = buf[SV_VY * SY_WG + o]; vyjpm = buf[SV_VYM * SY_WG + o];
       vzjp = buf[SV_VZ * SY_WG + o]; tjp = buf[SV_T * SY_WG + o];
     }
     if (jm_in) {
-      const int o = tid - 64;
+      const int o = tid - ST_SP;
       vxjm = buf[SV_VX * SY_WG + o]; vxjmp = buf[SV_VXP * SY_WG + o]; vyjm = buf[SV_VY * SY_WG + o];
       vzjm = buf[SV_VZ * SY_WG + o]; tjm = buf[SV_T * SY_WG + o]; prjm = buf[SV_PR * SY_WG + o];
       vzjmip = buf[SV_VZIP * SY_WG + o];
@@ -798,11 +801,11 @@ __global__ void k_divergence(RoutineArgs a) {
 // are read from (ux, uy, uz) and written to (vx, vy, vz): the u* buffers of
 // the substep path, or the fields themselves (in place) for the per-routine
 // entries and the refinement sweep.  The block shape is that of
-// k_stencil_rhs: SY_COLS waves on neighbouring y columns at the same 64
-// stored positions, marching through zplanes z planes with dph of planes
-// i-1, i, i+1 in registers; each wave publishes its plane-i dph in LDS for its
-// y neighbours (double-buffered, one barrier per plane), so only the group's
-// edge columns load a y neighbour.  Positions past k = nxm are not written.
+// k_stencil_rhs: ST_GC neighbouring y columns at the same ST_SP stored
+// positions, marching through zplanes z planes with dph of planes i-1, i, i+1
+// in registers; each thread publishes its plane-i dph in LDS for its y
+// neighbours (double-buffered, one barrier per plane), so only the group's
+// edge columns load a y neighbour.
 //
 // CFL = true (the last correction of a device step, not in place) also
 // reduces CalcMaxCFL.F90:26-41 over the corrected velocities into
@@ -816,12 +819,12 @@ __device__ __forceinline__ double corr(double u, double dd, double f) { return _
 template <bool CFL>
 __global__ void __launch_bounds__(SY_WG) k_correct(CorrectArgs a, int which, int zplanes) {
   extern __shared__ double sh[];
-  double *cr = sh;                    // [CZ_ROWS][64]
-  double *xv = sh + CZ_ROWS * 64;     // [2][2][SY_WG] published dph (and u*_y) of plane i
+  double *cr = sh;                    // [CZ_ROWS][ST_SP]
+  double *xv = sh + CZ_ROWS * ST_SP;  // [2][2][SY_WG] published dph (and u*_y) of plane i
   const Geom g = a.g;
   const int L = g.L, ldx = g.ldx, nxm = g.nxm;
-  const int nq = (ldx + 63) / 64;
-  const int ngr = (g.nym + SY_COLS - 1) / SY_COLS;
+  const int nq = (ldx + ST_SP - 1) / ST_SP;
+  const int ngr = (g.nym + ST_GC - 1) / ST_GC;
   const int nib = (g.nzm + zplanes - 1) / zplanes;
   const int64_t tile = xcd_block((int64_t)nib * ngr * nq);
   if (tile < 0 || (a.ds && coh_int(&a.ds->halted))) return;
@@ -829,8 +832,8 @@ __global__ void __launch_bounds__(SY_WG) k_correct(CorrectArgs a, int which, int
   const int64_t rest = tile / nq;
   const int gr = (int)(rest % ngr), ib = (int)(rest / ngr);
   const StepScalars c = a.ds ? coh_scalars(&a.ds->sc[a.ns - 1]) : a.sc;
-  for (int idx = threadIdx.x; idx < CZ_ROWS * 64; idx += SY_WG) {
-    const int r = idx / 64, sl = min(q * 64 + idx - r * 64, ldx - 1);
+  for (int idx = threadIdx.x; idx < CZ_ROWS * ST_SP; idx += SY_WG) {
+    const int r = idx / ST_SP, sl = min(q * ST_SP + idx - r * ST_SP, ldx - 1);
     int sp = sl;
     if (r == 5) {   // the position of k+1 (k+1 <= nx-1 holds for every stored k < nx-1)
       const int kk = k_of_pos(sl, L);
@@ -841,12 +844,12 @@ __global__ void __launch_bounds__(SY_WG) k_correct(CorrectArgs a, int which, int
     cr[idx] = a.kc[src * ldx + sp];
   }
   __syncthreads();
-  const int lane = (int)threadIdx.x & 63;
-  const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
-  const int ncols = min(SY_COLS, g.nym - gr * SY_COLS);
+  const int lane = (int)threadIdx.x & (ST_SP - 1);   // position within the tile
+  const int w = (int)threadIdx.x / ST_SP;             // column within the group
+  const int ncols = min(ST_GC, g.nym - gr * ST_GC);
   const bool colok = w < ncols;
-  const int jc = colok ? gr * SY_COLS + w : g.nym - 1;
-  const int sraw = q * 64 + lane;
+  const int jc = colok ? gr * ST_GC + w : g.nym - 1;
+  const int sraw = q * ST_SP + lane;
   const int s = min(sraw, ldx - 1);
   const int k = k_of_pos(s, L);
   const bool live = colok && sraw < ldx && k <= nxm - 1;
@@ -862,11 +865,11 @@ __global__ void __launch_bounds__(SY_WG) k_correct(CorrectArgs a, int which, int
   const double *__restrict__ dph = a.dph;
   const double *ux = a.ux, *uy = a.uy, *uz = a.uz;
   double *qx = a.vx, *qy = a.vy, *qz = a.vz, *__restrict__ qp = a.pr;
-  const double usukm = c.al * c.dt * cr[lane];
+  const double usukm = c.al * c.dt * cr[lane];   // row r of cr at cr[r * ST_SP + lane]
   const double udy = c.al * c.dt * c.dy, udz = c.al * c.dt * c.dz;
   const double be = c.al * c.beta;
-  const double amm = cr[64 + lane], acc = cr[128 + lane], app = cr[192 + lane];
-  const double u3m = cr[256 + lane], usukp = c.al * c.dt * cr[320 + lane];
+  const double amm = cr[ST_SP + lane], acc = cr[2 * ST_SP + lane], app = cr[3 * ST_SP + lane];
+  const double u3m = cr[4 * ST_SP + lane], usukp = c.al * c.dt * cr[5 * ST_SP + lane];
   const int i0 = ib * zplanes, i1 = min(i0 + zplanes, g.nzm);
   double dim_ = dph[col_off(jc, wrapm(i0 - 1, g.nzm), g) + s];
   double d = dph[col_off(jc, i0, g) + s];
@@ -899,9 +902,9 @@ __global__ void __launch_bounds__(SY_WG) k_correct(CorrectArgs a, int which, int
     buf[tid] = d;
     if (CFL) buf[SY_WG + tid] = vyo;
     __syncthreads();
-    if (jm_in) djm = buf[tid - 64];
-    if (jp_in) djp = buf[tid + 64];
-    if (CFL && jp_in) uyjp = buf[SY_WG + tid + 64];
+    if (jm_in) djm = buf[tid - ST_SP];
+    if (jp_in) djp = buf[tid + ST_SP];
+    if (CFL && jp_in) uyjp = buf[SY_WG + tid + ST_SP];
     par ^= 1;
     if (stv) {
       if (which & 1) {
@@ -933,11 +936,11 @@ __global__ void __launch_bounds__(SY_WG) k_correct(CorrectArgs a, int which, int
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) cflm = fmax(cflm, __shfl_xor(cflm, off, 64));
     __syncthreads();   // every wave is past its last read of xv
-    if (lane == 0) xv[w] = cflm;
+    if ((threadIdx.x & 63) == 0) xv[threadIdx.x >> 6] = cflm;
     __syncthreads();
     if (threadIdx.x == 0) {
       double m = xv[0];
-      for (int q = 1; q < SY_COLS; q++) m = fmax(m, xv[q]);
+      for (int q = 1; q < SY_WG / 64; q++) m = fmax(m, xv[q]);
       atomicMax(a.cflbits, (unsigned long long)__double_as_longlong(m));
     }
   }
@@ -985,8 +988,8 @@ static dim3 column_grid(const Geom &g) {
 hipError_t launch_stencil_rhs(const FusedArgs &a, hipStream_t st) {
   const int64_t n = a.g.ncol * a.g.ldx;
   const int zplanes = a.zm > 0 ? a.zm : ZM_DEFAULT;
-  const int64_t ntile = (int64_t)((a.g.nzm + zplanes - 1) / zplanes) * ((a.g.nym + SY_COLS - 1) / SY_COLS) *
-                        ((a.g.ldx + 63) / 64);
+  const int64_t ntile = (int64_t)((a.g.nzm + zplanes - 1) / zplanes) * ((a.g.nym + ST_GC - 1) / ST_GC) *
+                        ((a.g.ldx + ST_SP - 1) / ST_SP);
   const dim3 grid((unsigned)(((ntile + 7) / 8) * 8));
   const size_t shm = stencil_lds(a.g);
   const bool small = (uint64_t)n * sizeof(double) < (1ull << 32);
@@ -1051,9 +1054,9 @@ hipError_t launch_divergence(const RoutineArgs &a, hipStream_t st) {
 }
 hipError_t launch_correct(const CorrectArgs &a, int which, hipStream_t st) {
   const int zplanes = ZM_DEFAULT;
-  const int64_t nt = (int64_t)((a.g.nzm + zplanes - 1) / zplanes) * ((a.g.nym + SY_COLS - 1) / SY_COLS) *
-                     ((a.g.ldx + 63) / 64);
-  const size_t shm = sizeof(double) * (CZ_ROWS * 64 + 2 * 2 * SY_WG);
+  const int64_t nt = (int64_t)((a.g.nzm + zplanes - 1) / zplanes) * ((a.g.nym + ST_GC - 1) / ST_GC) *
+                     ((a.g.ldx + ST_SP - 1) / ST_SP);
+  const size_t shm = sizeof(double) * (CZ_ROWS * ST_SP + 2 * 2 * SY_WG);
   const dim3 grid((unsigned)(((nt + 7) / 8) * 8));
   if (a.cflbits) {
     if (which != 3 || a.ux == a.vx) return hipErrorInvalidValue;   // the fold needs u* apart from the fields
@@ -1064,7 +1067,7 @@ hipError_t launch_correct(const CorrectArgs &a, int which, hipStream_t st) {
   return hipGetLastError();
 }
 
-// LDS per block: WG*L doubles (per-routine solve), SC_N*64 + 2*SV_N*SY_WG (stencil
+// LDS per block: WG*L doubles (per-routine solve), SC_N*ST_SP + 2*SV_N*SY_WG (stencil
 // pass), SYS_N*RW_N*16*L (column solve); L <= 80 keeps all within gfx950's 160 KiB.
 int max_supported_L() { return 80; }
 size_t column_lds_need(const Geom &g) {
