@@ -66,11 +66,14 @@ __device__ __forceinline__ void for_points(const Geom &g, int kmax, F &&f) {
   }
 }
 
-// CalcMaxCFL (CalcMaxCFL.F90:26-41) in the block shape of the stencil pass:
-// four waves on four neighbouring y columns at the same 64 stored positions
-// march through CFL_ZP z planes, with vz of plane i+1 carried in registers to
-// the next plane and vy of column j+1 taken from the neighbouring wave through
-// LDS (the group's last column loads it), so each field is read about once.
+// CalcMaxCFL (CalcMaxCFL.F90:26-41): four waves on four neighbouring y columns
+// at the same 64 stored positions march through CFL_ZP z planes, with vz of
+// plane i+1 carried in registers to the next plane and vy of column j+1 taken
+// from the neighbouring wave through LDS (the group's last column loads it),
+// so each field is read about once.  (The stencil pass's block of sixteen
+// columns at one 128-byte line each ran slower here, 1.34-1.38 against
+// 1.15 ms at 384x768x768: these blocks are persistent and walk their tiles
+// one after another.)
 // Each of the RBLK blocks walks a contiguous range of these tiles (XCD-
 // remapped) and leaves one maximum; the terms are summed in the reference
 // order, so the maximum equals that of the point-wise form bit for bit.
