@@ -416,9 +416,10 @@ def test_device_steps_stop_after_error_code(afid):
 
 # Sizes away from the small cases: column-solve instances for larger chunk
 # counts (L = 25, 44, 69, 80 -> LM 25, 48, 80, 80), x pitches above 512
-# positions (several positions per thread in the correction and reduction
-# kernels, several parts in the stencil pass), a y line of 768 (half length
-# 384: LDS above 64 KiB) and a z line of 750 (radices 2, 3, 5, 5, 5).
+# positions (up to 80 position tiles per column in the stencil and
+# correction passes, several positions per thread in the reduction kernels),
+# a y line of 768 (half length 384: LDS above 64 KiB) and a z line of 750
+# (radices 2, 3, 5, 5, 5).
 EDGE_CASES = [
     dict(nxm=384, nym=8, nzm=6, istr3=6, str3=6.0, ylen=1.0, zlen=1.0, ray=1e6, pra=1.0),
     dict(nxm=700, nym=6, nzm=10, istr3=6, str3=6.0, ylen=1.0, zlen=1.0, ray=1e6, pra=1.0),
@@ -426,9 +427,11 @@ EDGE_CASES = [
     dict(nxm=1279, nym=4, nzm=4, istr3=6, str3=6.0, ylen=1.0, zlen=1.0, ray=1e6, pra=1.0),
     dict(nxm=12, nym=768, nzm=6, istr3=6, str3=6.0, ylen=1.0, zlen=1.0, ray=1e6, pra=1.0),
     dict(nxm=10, nym=6, nzm=750, istr3=4, str3=3.0, ylen=1.0, zlen=1.0, ray=1e6, pra=1.0),
-    # fewer y columns than the stencil pass groups per block (4), and a ragged last group
+    # fewer y columns than a stencil or correction block holds (16), and five
+    # full groups followed by a group of one column (81 = 3^4, an odd y line)
     dict(nxm=40, nym=3, nzm=8, istr3=6, str3=6.0, ylen=1.0, zlen=1.0, ray=1e6, pra=1.0),
     dict(nxm=24, nym=5, nzm=9, istr3=4, str3=3.0, ylen=1.0, zlen=1.0, ray=1e6, pra=1.0),
+    dict(nxm=20, nym=81, nzm=6, istr3=6, str3=6.0, ylen=1.0, zlen=1.0, ray=1e6, pra=1.0),
 ]
 
 
