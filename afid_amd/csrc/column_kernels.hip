@@ -209,22 +209,6 @@ template <typename OffT>
 __device__ __forceinline__ double ldf(const double *base, OffT boff) {
   return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(base) + boff);
 }
-template <typename OffT>
-__device__ __forceinline__ void stf(double *base, OffT boff, double v) {
-  *reinterpret_cast<double *>(reinterpret_cast<char *>(base) + boff) = v;
-}
-// History arrays (rux..rutemp): FP64, or FP32 in the mixed-precision mode (H32),
-// where element e sits at byte 4e, half the FP64 byte offset.
-template <bool H32, typename OffT>
-__device__ __forceinline__ double ldh(const double *base, OffT boff) {
-  if (H32) return (double)*reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + (boff >> 1));
-  return ldf(base, boff);
-}
-template <bool H32, typename OffT>
-__device__ __forceinline__ void sth(double *base, OffT boff, double v) {
-  if (H32) *reinterpret_cast<float *>(reinterpret_cast<char *>(base) + (boff >> 1)) = (float)v;
-  else stf(base, boff, v);
-}
 
 // Non-temporal stores (streamed past the caches' normal retention): the
 // stencil pass's b' and history stores and the column solve's new fields,
@@ -234,10 +218,22 @@ template <typename OffT>
 __device__ __forceinline__ void stf_nt(double *base, OffT boff, double v) {
   __builtin_nontemporal_store(v, reinterpret_cast<double *>(reinterpret_cast<char *>(base) + boff));
 }
+// History arrays (rux..rutemp): FP64, or FP32 in the mixed-precision mode (H32),
+// where element e sits at byte 4e, half the FP64 byte offset.
 template <bool H32, typename OffT>
 __device__ __forceinline__ void sth_nt(double *base, OffT boff, double v) {
   if (H32) __builtin_nontemporal_store((float)v, reinterpret_cast<float *>(reinterpret_cast<char *>(base) + (boff >> 1)));
   else stf_nt(base, boff, v);
+}
+// History loads of the stencil pass (read once per substep, by that pass
+// alone) stream past the caches too, leaving the L2 to the field lines that
+// neighbouring tiles read again: 3 % less fetched, about 1 % faster at
+// 384x768x768.
+template <bool H32, typename OffT>
+__device__ __forceinline__ double ldh_nt(const double *base, OffT boff) {
+  if (H32)
+    return (double)__builtin_nontemporal_load(reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + (boff >> 1)));
+  return __builtin_nontemporal_load(reinterpret_cast<const double *>(reinterpret_cast<const char *>(base) + boff));
 }
 
 // ---------------------------------------------------------------- substep update
@@ -259,10 +255,9 @@ __device__ __forceinline__ void sth_nt(double *base, OffT boff, double v) {
 // holds ldx = 16 L stored positions, so its L tiles are all full.  Blocks are
 // ordered position-range fastest, then y group, then z range (XCD-remapped),
 // so the k+-1 lines a tile reads belong to tiles running on the same XCD at
-// the same time.  A plane's loads issue in
-// one batch before any value is used; the history rows outside the reference
-// loops (k = 1 and nx for vx/temp, nx for vy/vz), which nothing reads, are
-// kept at 0.  Histories that no later substep reads (keep_ru = 0: RK3
+// the same time.  A plane's loads issue in one batch before any value is
+// used; the history rows outside the reference loops (k = 1 and nx for
+// vx/temp, nx for vy/vz), which nothing reads, are kept at 0.  Histories that no later substep reads (keep_ru = 0: RK3
 // substep 3, since the next step's substep 1 has ro = 0) are not stored.
 // k_column_solve: a wave owns four columns (16 chunk lanes each) and solves the
 // four systems in registers: A q_new = b', with the factor rows staged in LDS.
@@ -395,8 +390,8 @@ __global__ void __launch_bounds__(SY_WG)
       vzjm = ldf(vz, CJM + sb); tjm = ldf(T, CJM + sb); prjm = ldf(pr, CJM + sb);
       vzjmip = ldf(vz, colb(jm, ip) + sb);
     }
-    const double rxo = USE_RU ? ldh<H32>(rux, C + sb) : 0.0, ryo = USE_RU ? ldh<H32>(ruy, C + sb) : 0.0;
-    const double rzo = USE_RU ? ldh<H32>(ruz, C + sb) : 0.0, rto = USE_RU ? ldh<H32>(rut, C + sb) : 0.0;
+    const double rxo = USE_RU ? ldh_nt<H32>(rux, C + sb) : 0.0, ryo = USE_RU ? ldh_nt<H32>(ruy, C + sb) : 0.0;
+    const double rzo = USE_RU ? ldh_nt<H32>(ruz, C + sb) : 0.0, rto = USE_RU ? ldh_nt<H32>(rut, C + sb) : 0.0;
     const double st0 = coh_dbl(stalep + (int64_t)ic * g.nym + jc);
     const double stale0 = k == 0 ? st0 : 0.0;
     double *buf = xv + par * (SV_N * SY_WG);
