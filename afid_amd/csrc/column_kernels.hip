@@ -225,14 +225,14 @@ __device__ __forceinline__ void sth_nt(double *base, OffT boff, double v) {
   if (H32) __builtin_nontemporal_store((float)v, reinterpret_cast<float *>(reinterpret_cast<char *>(base) + (boff >> 1)));
   else stf_nt(base, boff, v);
 }
-// History loads of the stencil pass (read once per substep, by that pass
+// FP64 history loads of the stencil pass (read once per substep, by that pass
 // alone) stream past the caches too, leaving the L2 to the field lines that
 // neighbouring tiles read again: 3 % less fetched, about 1 % faster at
-// 384x768x768.
+// 384x768x768.  FP32 histories, whose tile rows are 64-byte half lines, load
+// normally (streamed, that pass ran 4 % slower).
 template <bool H32, typename OffT>
 __device__ __forceinline__ double ldh_nt(const double *base, OffT boff) {
-  if (H32)
-    return (double)__builtin_nontemporal_load(reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + (boff >> 1)));
+  if (H32) return (double)*reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + (boff >> 1));
   return __builtin_nontemporal_load(reinterpret_cast<const double *>(reinterpret_cast<const char *>(base) + boff));
 }
 
