@@ -220,9 +220,12 @@ __device__ __forceinline__ void stf_nt(double *base, OffT boff, double v) {
 }
 // History arrays (rux..rutemp): FP64, or FP32 in the mixed-precision mode (H32),
 // where element e sits at byte 4e, half the FP64 byte offset.
+// FP32 histories store normally: a tile row of 16 positions is half a 128-byte
+// line, whose other half the neighbouring tile writes, and the halves merge in
+// the L2 only when the stores are ordinary.
 template <bool H32, typename OffT>
 __device__ __forceinline__ void sth_nt(double *base, OffT boff, double v) {
-  if (H32) __builtin_nontemporal_store((float)v, reinterpret_cast<float *>(reinterpret_cast<char *>(base) + (boff >> 1)));
+  if (H32) *reinterpret_cast<float *>(reinterpret_cast<char *>(base) + (boff >> 1)) = (float)v;
   else stf_nt(base, boff, v);
 }
 // FP64 history loads of the stencil pass (read once per substep, by that pass
