@@ -395,7 +395,7 @@ __global__ void __launch_bounds__(SY_WG)
     }
     const double rxo = USE_RU ? ldh_nt<H32>(rux, C + sb) : 0.0, ryo = USE_RU ? ldh_nt<H32>(ruy, C + sb) : 0.0;
     const double rzo = USE_RU ? ldh_nt<H32>(ruz, C + sb) : 0.0, rto = USE_RU ? ldh_nt<H32>(rut, C + sb) : 0.0;
-    const double st0 = coh_dbl(stalep + (int64_t)ic * g.nym + jc);
+    const double st0 = q == 0 ? coh_dbl(stalep + (int64_t)ic * g.nym + jc) : 0.0;   // k = 0 lies in row 0
     const double stale0 = k == 0 ? st0 : 0.0;
     double *buf = xv + par * (SV_N * SY_WG);
     buf[SV_VX * SY_WG + tid] = cur.vxc;
